@@ -1,0 +1,795 @@
+// ctrecon.hip -- gfx950 kernels and C ABI for the distance-driven CT projector pair and FDK.
+//
+// Kernels (all FP32 arithmetic; FP64 only in the per-view geometry kernels):
+//   fp_geometry_kernel     per-view DD footprint tables, FP64 -> FP32 (index space)
+//   transpose_xy_kernel    vol[z][y][x] -> vol_t[z][x][y] for x-slab views (coalesced gathers)
+//   dd_forward_kernel      one work-item per (view, detector cell); slabs looped in registers
+//   bp_geometry_kernel     per-view backprojection constants, FP64 -> FP32
+//   dd_backproject_kernel  one work-item per voxel column chunk; all views looped in registers
+//   spectrum_kernel        filter magnitude from the Ram-Lak cosine series, FP64
+//   twiddle_kernel         exp(-2 pi i k / N) table, FP64 -> FP32
+//   fdk_filter_kernel      cosine weight + LDS Stockham FFT convolution, two rows per FFT
+//
+// Reference semantics are cited at each kernel (leehoy/CTReconstruction paths).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+
+#include "ctrecon.h"
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+inline size_t align_up(size_t n, size_t a) { return (n + a - 1) / a * a; }
+
+// ------------------------------------------------------------------------------------------
+// Host-side validation
+// ------------------------------------------------------------------------------------------
+
+int check_geometry(const ctr_geometry* g) {
+    if (!g) return CTR_ERR_INVALID;
+    if (g->nx <= 0 || g->ny <= 0 || g->nz <= 0 || g->nu <= 0 || g->nv <= 0 || g->n_views <= 0)
+        return CTR_ERR_INVALID;
+    if (!(g->dx > 0) || !(g->dy > 0) || !(g->dz > 0) || !(g->du > 0) || !(g->dv > 0))
+        return CTR_ERR_INVALID;
+    if (!(g->sad > 0) || !(g->sdd > 0)) return CTR_ERR_INVALID;
+    if (g->detector_shape != CTR_DETECTOR_FLAT && g->detector_shape != CTR_DETECTOR_CURVED)
+        return CTR_ERR_UNSUPPORTED;
+    // flat indices are 32-bit inside the kernels
+    const double vox = (double)g->nx * g->ny * g->nz;
+    const double cells = (double)g->n_views * g->nv * g->nu;
+    if (vox >= 2147483647.0 || cells >= 2147483647.0) return CTR_ERR_INVALID;
+    // grid y/z limits: forward (nv/4, n_views), transpose (ny/32, nz), backprojection (ny/4, nz/8)
+    if (g->n_views > 65535 || g->nv > 65535 * 4 || g->ny > 65535 * 4 || g->nz > 65535)
+        return CTR_ERR_INVALID;
+    return CTR_OK;
+}
+
+int status_from_hip(hipError_t e) { return e == hipSuccess ? CTR_OK : CTR_ERR_HIP; }
+
+// ------------------------------------------------------------------------------------------
+// Forward projection
+// ------------------------------------------------------------------------------------------
+
+// Per-view table layout (float32, stride fp_view_stride(nu)):
+//   [0,        nu+1)   alpha_e : transverse index coordinate of U-edge line e at slab 0
+//   [nu+1,   2nu+2)    beta_e  : its change per slab
+//   [2nu+2,  3nu+2)    T0_j    : (s_0 - S_s) / (C_s[j] - S_s), fraction source->column j at slab 0
+//   [3nu+2,  4nu+2)    dT_j    : change of T per slab
+//   [4nu+2,  5nu+2)    sec_j   : |d_s| * sqrt(1 + s1_j^2)
+//   [5nu+2,  6nu+2)    inv_j   : 1 / (C_s[j] - S_s)
+//   [6nu+2]            gamma   : (S_z - ReconCenter_z)/(-dz) + nz/2
+// Row table (float32): [0, nv+1) nu_k = V2[k]/(-dz); [nv+1, 2nv+1) Vc[k] (mm).
+inline int fp_view_stride(int nu) { return (int)align_up((size_t)(6 * nu + 3), 64); }
+
+struct FpLayout {
+    size_t tabs_off, rows_off, volt_off, total;
+};
+
+FpLayout fp_layout(const ctr_geometry* g) {
+    FpLayout L;
+    L.tabs_off = 0;
+    L.rows_off = align_up(L.tabs_off + (size_t)fp_view_stride(g->nu) * g->n_views * 4, 256);
+    L.volt_off = align_up(L.rows_off + (size_t)(2 * g->nv + 1) * 4, 256);
+    L.total = align_up(L.volt_off + (size_t)g->nx * g->ny * g->nz * 4, 256);
+    return L;
+}
+
+// Detector lattice of FlatDetectorConstruction / CurvedDetectorConstruction
+// (python/Reconstruction_pycuda.py:113-168): in-plane position of a detector point at
+// transverse coordinate u (mm, or radians on the curved arc) for a view with source S.
+__device__ inline void lattice_xy(const ctr_geometry& g, double c, double s, double Sx, double Sy,
+                                  double u, double* px, double* py) {
+    // eu = (cos, sin, 0), ew = (sin, -cos, 0)
+    if (g.detector_shape == CTR_DETECTOR_FLAT) {
+        *px = Sx + u * c + g.sdd * s;
+        *py = Sy + u * s - g.sdd * c;
+    } else {
+        double su, cu;
+        sincos(u, &su, &cu);
+        *px = Sx + g.sdd * su * c + g.sdd * cu * s;
+        *py = Sy + g.sdd * su * s - g.sdd * cu * c;
+    }
+}
+
+// One thread per (edge e in [0, nu], view).  FP64 throughout; rounded to FP32 at the store.
+// Follows the host geometry of distance_forward (python/Reconstruction_pycuda.py:828-864,
+// 924-935): lines through the source and the cell-edge lattice, intersection length L.
+__global__ void fp_geometry_kernel(ctr_geometry g, const double* __restrict__ angles,
+                                   const int32_t* __restrict__ branch, float* __restrict__ tabs,
+                                   float* __restrict__ rows, int stride) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int view = blockIdx.y;
+    const int nu = g.nu, nv = g.nv;
+    if (view == 0 && e <= nv) {
+        // row edges / centres: flat lattice  v2 = -(k - (nv-1)/2) dv + dv/2 (:160),
+        //                      curved lattice v2 = -(k - (nv-1)/2) dv - dv/2 (:129)
+        const double half = (g.detector_shape == CTR_DETECTOR_FLAT) ? 0.5 * g.dv : -0.5 * g.dv;
+        const double v2 = -((double)e - (nv - 1) / 2.0) * g.dv + half + g.det_offset[1];
+        rows[e] = (float)(v2 / (-g.dz));
+        if (e < nv) rows[nv + 1 + e] = (float)(-((double)e - (nv - 1) / 2.0) * g.dv + g.det_offset[1]);
+    }
+    if (e > nu) return;
+    const double th = angles[view];
+    double s, c;
+    sincos(th, &s, &c);
+    const double Sx = -g.sad * s, Sy = g.sad * c;
+    const double Sz = g.source_init[2] + g.helical * th / (2.0 * kPi);
+    const int br = branch[view];
+    float* T = tabs + (size_t)view * stride;
+    if (br != CTR_BRANCH_X && br != CTR_BRANCH_Y) {  // z-slabs: rejected on the host
+        if (e < nu) { T[2 * nu + 2 + e] = 0.f; }
+        return;
+    }
+    const bool xs = (br == CTR_BRANCH_X);
+    // slab axis s / transverse axis t; signed steps as the reference (dy, dz negated)
+    const double ds = xs ? g.dx : -g.dy;
+    const double dt = xs ? -g.dy : g.dx;
+    const int ns = xs ? g.nx : g.ny;
+    const int nt = xs ? g.ny : g.nx;
+    const double rcs = xs ? g.recon_center[0] : g.recon_center[1];
+    const double rct = xs ? g.recon_center[1] : g.recon_center[0];
+    const double Ss = xs ? Sx : Sy;
+    const double St = xs ? Sy : Sx;
+    const double s0 = rcs - (ns - 1) / 2.0 * ds;  // slab 0 centre: Plane[0] + d/2
+    const double step = (g.detector_shape == CTR_DETECTOR_FLAT) ? g.du : g.du / g.sdd;
+    {
+        // edge e: u2 = (e - (nu-1)/2) du - du/2 + offset
+        const double u2 = ((double)e - (nu - 1) / 2.0) * step - step / 2.0 + g.det_offset[0];
+        double bx, by;
+        lattice_xy(g, c, s, Sx, Sy, u2, &bx, &by);
+        const double Bs = xs ? bx : by, Bt = xs ? by : bx;
+        const double m = (Bt - St) / (Bs - Ss);
+        const double t0 = St + m * (s0 - Ss);
+        T[e] = (float)((t0 - rct) / dt + nt / 2.0);
+        T[nu + 1 + e] = (float)(ds * m / dt);
+    }
+    if (e < nu) {
+        const double u = ((double)e - (nu - 1) / 2.0) * step + g.det_offset[0];
+        double cx, cy;
+        lattice_xy(g, c, s, Sx, Sy, u, &cx, &cy);
+        const double Cs = xs ? cx : cy, Ct = xs ? cy : cx;
+        const double inv = 1.0 / (Cs - Ss);
+        T[2 * nu + 2 + e] = (float)((s0 - Ss) * inv);
+        T[3 * nu + 2 + e] = (float)(ds * inv);
+        const double s1 = (St - Ct) / (Ss - Cs);
+        T[4 * nu + 2 + e] = (float)(fabs(ds) * sqrt(1.0 + s1 * s1));
+        T[5 * nu + 2 + e] = (float)inv;
+    }
+    if (e == 0) T[6 * nu + 2] = (float)((Sz - g.recon_center[2]) / (-g.dz) + g.nz / 2.0);
+}
+
+// vol[z][y][x] -> vol_t[z][x][y]; 32x32 tiles through LDS (padded against bank conflicts).
+__global__ void __launch_bounds__(256) transpose_xy_kernel(const float* __restrict__ in,
+                                                           float* __restrict__ out, int nx, int ny) {
+    __shared__ float tile[32][33];
+    const size_t zoff = (size_t)blockIdx.z * nx * ny;
+    const int x0 = blockIdx.x * 32, y0 = blockIdx.y * 32;
+    for (int r = threadIdx.y; r < 32; r += 8) {
+        const int y = y0 + r, x = x0 + threadIdx.x;
+        if (x < nx && y < ny) tile[r][threadIdx.x] = in[zoff + (size_t)y * nx + x];
+    }
+    __syncthreads();
+    for (int r = threadIdx.y; r < 32; r += 8) {
+        const int x = x0 + r, y = y0 + threadIdx.x;
+        if (x < nx && y < ny) out[zoff + (size_t)x * ny + y] = tile[threadIdx.x][r];
+    }
+}
+
+// Overlap weight (index space) of cell k with the footprint [lo, hi]; the cases
+// (single cell -> 1, first, last, interior) are those of python/GPUFuncs_pycuda.py:427-452.
+__device__ inline float dd_weight(int k, int k0, int k1, float lo, float hi, float inv_w,
+                                  float zero_tol) {
+    float w;
+    if (k0 == k1) {
+        w = 1.0f;
+    } else {
+        w = (fminf(hi, (float)(k + 1)) - fmaxf(lo, (float)k)) * inv_w;
+    }
+    return (w < zero_tol) ? 0.0f : w;
+}
+
+// DD forward projection, one work-item per (view, detector cell).  Kernel semantics of
+// distance_project_on_{y,x}3 (python/GPUFuncs_pycuda.py:398-524): for each slab the cell's
+// four boundary lines give a footprint [a1,a2] x [c1,c2] that is overlapped with the voxel
+// grid; here the reference's per-(slab, cell) atomicAdd is replaced by a register sum over
+// slabs and one coalesced store.  Coordinates are in index space (plane k at k), so the
+// same code serves increasing (x) and decreasing (y, z) planes.
+__global__ void __launch_bounds__(256) dd_forward_kernel(
+    const float* __restrict__ vol, const float* __restrict__ vol_t, float* __restrict__ proj,
+    const float* __restrict__ tabs, const float* __restrict__ rows,
+    const int32_t* __restrict__ branch, int nx, int ny, int nz, int nu, int nv, int stride,
+    float zero_tol) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y;
+    const int view = blockIdx.z;
+    if (j >= nu || k >= nv) return;
+    const int br = branch[view];
+    const size_t out_idx = ((size_t)view * nv + k) * nu + j;
+    if (br != CTR_BRANCH_X && br != CTR_BRANCH_Y) {
+        proj[out_idx] = 0.0f;
+        return;
+    }
+    const bool xs = (br == CTR_BRANCH_X);
+    const float* __restrict__ src = xs ? vol_t : vol;
+    const int ns = xs ? nx : ny;
+    const int nt = xs ? ny : nx;
+    const int s_stride = xs ? ny : nx;
+    const int v_stride = nx * ny;
+    const float* T = tabs + (size_t)view * stride;
+    const float a_al = T[j], b_al = T[j + 1];
+    const float a_be = T[nu + 1 + j], b_be = T[nu + 2 + j];
+    const float T0 = T[2 * nu + 2 + j], dT = T[3 * nu + 2 + j];
+    const float sec = T[4 * nu + 2 + j], inv = T[5 * nu + 2 + j];
+    const float gam = T[6 * nu + 2];
+    // V1 uses edge row k+1, V2 edge row k (python/Reconstruction_pycuda.py:844-846)
+    const float nu1 = rows[k + 1], nu2 = rows[k];
+    const float vc = rows[nv + 1 + k];
+    const float s2 = vc * inv;
+    const float Lk = sec * sqrtf(1.0f + s2 * s2);
+
+    float acc = 0.0f;
+    for (int n = 0; n < ns; ++n) {
+        const float fn = (float)n;
+        const float a1 = fmaf(a_be, fn, a_al);
+        const float a2 = fmaf(b_be, fn, b_al);
+        const float t = fmaf(dT, fn, T0);
+        const float c1 = fmaf(nu1, t, gam);
+        const float c2 = fmaf(nu2, t, gam);
+        const float alo = fminf(a1, a2), ahi = fmaxf(a1, a2);
+        const float clo = fminf(c1, c2), chi = fmaxf(c1, c2);
+        const int ka0 = (int)floorf(alo), ka1 = (int)floorf(ahi);
+        const int kc0 = (int)floorf(clo), kc1 = (int)floorf(chi);
+        const int kas = max(ka0, 0), kae = min(ka1, nt - 1);
+        const int kcs = max(kc0, 0), kce = min(kc1, nz - 1);
+        if (kas > kae || kcs > kce) continue;
+        const float inv_a = 1.0f / (ahi - alo);
+        const float inv_c = 1.0f / (chi - clo);
+        const float* slab = src + (size_t)n * s_stride;
+        float sum = 0.0f;
+        for (int kc = kcs; kc <= kce; ++kc) {
+            const float wc = dd_weight(kc, kc0, kc1, clo, chi, inv_c, zero_tol);
+            const float* line = slab + (size_t)kc * v_stride;
+            float row = 0.0f;
+            for (int ka = kas; ka <= kae; ++ka) {
+                const float wa = dd_weight(ka, ka0, ka1, alo, ahi, inv_a, zero_tol);
+                row = fmaf(line[ka], wa, row);
+            }
+            sum = fmaf(row, wc, sum);
+        }
+        acc += sum;
+    }
+    proj[out_idx] = acc * Lk;
+}
+
+// ------------------------------------------------------------------------------------------
+// Backprojection
+// ------------------------------------------------------------------------------------------
+
+constexpr int kBpViewStride = 16;  // floats per view in the backprojection table
+constexpr int kBpZChunk = 8;       // voxels along z per work-item
+
+// Per-view constants of flat_distance_backproj_arb2's param vector
+// (python/Reconstruction_pycuda.py:405-407, 421-423), in FP64, stored as FP32:
+//   [0] cos th  [1] sin th  [2] Sx  [3] Sy  [4] Sz
+//   [5] Au [6] Bu : u index = Au - Bu * slope_u          (flat)
+//                   u index = Au - Bu * atan(slope_u)    (curved, numba units)
+//   [7] Av [8] Bv : v index = Av - Bv * slope_v
+//   [9] SAD
+__global__ void bp_geometry_kernel(ctr_geometry g, const double* __restrict__ angles,
+                                   float* __restrict__ vt, int mode) {
+    const int view = blockIdx.x * blockDim.x + threadIdx.x;
+    if (view >= g.n_views) return;
+    const double th = angles[view];
+    double s, c;
+    sincos(th, &s, &c);
+    const double Sx = g.source_init[0], Sy = g.source_init[1];
+    const double Sz = g.source_init[2] + g.helical * th / (2.0 * kPi);
+    const double Dy = g.detector_init[1];
+    const double du = (g.detector_shape == CTR_DETECTOR_FLAT) ? g.du : g.du / g.sdd;
+    const double dv = -g.dv;
+    // reference grid (:353-354) or true edges (:356-357)
+    const double cu = (mode == CTR_BP_FDK_EDGE) ? g.nu / 2.0 : (g.nu - 1) / 2.0;
+    const double cv = (mode == CTR_BP_FDK_EDGE) ? g.nv / 2.0 : (g.nv - 1) / 2.0;
+    const double ki0 = -cu * du + g.det_offset[0];
+    const double p0 = -cv * dv + g.det_offset[1];
+    // coord = slope * Dy + (S - slope * Sy) = S - slope * (Sy - Dy)
+    const double span = Sy - Dy;
+    float* T = vt + (size_t)view * kBpViewStride;
+    T[0] = (float)c;
+    T[1] = (float)s;
+    T[2] = (float)Sx;
+    T[3] = (float)Sy;
+    T[4] = (float)Sz;
+    if (g.detector_shape == CTR_DETECTOR_FLAT) {
+        T[5] = (float)((Sx - ki0) / du);
+        T[6] = (float)(span / du);
+    } else {
+        T[5] = (float)(-ki0 / du);
+        T[6] = (float)(1.0 / du);
+    }
+    T[7] = (float)((Sz - p0) / dv);
+    T[8] = (float)(span / dv);
+    T[9] = (float)g.sad;
+}
+
+// FDK-weighted DD backprojection.  Kernel semantics of flat_distance_backproj_arb2
+// (python/GPUFuncs_pycuda.py:837-956) and curved_distance_backproj_arb
+// (python/GPUFuncs_numba.py:537-685): the 8 voxel corners are rotated by the view angle, the
+// min/max slopes through the unrotated source give the u- and v-footprints on the detector
+// plane, which are overlapped with the detector grid and weighted by SAD^2/(SAD - y_c)^2.
+// With angle2 = 0 the u-footprint depends only on the voxel column, so it is computed once
+// per (column, view) and reused along z; the reference's per-tap atomicAdd becomes a
+// register accumulation over all views and one store.
+__global__ void __launch_bounds__(256) dd_backproject_kernel(
+    const float* __restrict__ q, float* __restrict__ vol, const float* __restrict__ vt,
+    int nx, int ny, int nz, int nu, int nv, int n_views, float x0, float y0, float z0,
+    float dx, float dy, float dz, int curved, float scale, float zero_tol, int accumulate) {
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+    const int iz0 = blockIdx.z * kBpZChunk;
+    if (ix >= nx || iy >= ny) return;
+    // voxel centre (python/Reconstruction_pycuda.py:350-352); dy, dz are the signed steps
+    const float X = fmaf((float)ix, dx, x0);
+    const float Y = fmaf((float)iy, dy, y0);
+    const float hx = 0.5f * fabsf(dx), hy = 0.5f * fabsf(dy), hz = 0.5f * fabsf(dz);
+    float acc[kBpZChunk];
+#pragma unroll
+    for (int i = 0; i < kBpZChunk; ++i) acc[i] = 0.0f;
+
+    for (int view = 0; view < n_views; ++view) {
+        const float* T = vt + (size_t)view * kBpViewStride;
+        const float c = T[0], s = T[1], Sx = T[2], Sy = T[3], Sz = T[4];
+        const float Au = T[5], Bu = T[6], Av = T[7], Bv = T[8], R = T[9];
+        const float xr0 = X * c + Y * s;
+        const float yr0 = Y * c - X * s;
+        // four in-plane corners (the z pairs share them)
+        float smin = 3.0e38f, smax = -3.0e38f, rmin = 3.0e38f, rmax = -3.0e38f;
+#pragma unroll
+        for (int corner = 0; corner < 4; ++corner) {
+            const float sx = (corner & 1) ? -hx : hx;
+            const float sy = (corner & 2) ? -hy : hy;
+            const float xr = xr0 + sx * c + sy * s;
+            const float yr = yr0 + sy * c - sx * s;
+            const float r = 1.0f / (Sy - yr);
+            const float su = (Sx - xr) * r;
+            smin = fminf(smin, su);
+            smax = fmaxf(smax, su);
+            rmin = fminf(rmin, r);
+            rmax = fmaxf(rmax, r);
+        }
+        float fu1, fu2;
+        if (curved) {
+            fu1 = Au - Bu * atanf(smin);
+            fu2 = Au - Bu * atanf(smax);
+        } else {
+            fu1 = Au - Bu * smin;
+            fu2 = Au - Bu * smax;
+        }
+        const float ulo = fminf(fu1, fu2), uhi = fmaxf(fu1, fu2);
+        const int ku0 = (int)floorf(ulo), ku1 = (int)floorf(uhi);
+        const int kus = max(ku0, 0), kue = min(ku1, nu - 1);
+        if (kus > kue) continue;
+        const float inv_u = 1.0f / (uhi - ulo);
+        const float dd = R - yr0;
+        const float W = scale * (R * R) / (dd * dd);
+        const float* qv = q + (size_t)view * nv * nu;
+#pragma unroll
+        for (int i = 0; i < kBpZChunk; ++i) {
+            const int iz = iz0 + i;
+            if (iz >= nz) break;
+            const float Z = fmaf((float)iz, dz, z0);
+            const float e = Sz - Z;
+            // slope_v = (Sz - z') / (Sy - y') over the 8 corners: extremes at r_min / r_max
+            const float ep = e + hz, em = e - hz;
+            const float vmax = ep * (ep >= 0.0f ? rmax : rmin);
+            const float vmin = em * (em >= 0.0f ? rmin : rmax);
+            const float fv1 = Av - Bv * vmin;
+            const float fv2 = Av - Bv * vmax;
+            const float vlo = fminf(fv1, fv2), vhi = fmaxf(fv1, fv2);
+            const int kv0 = (int)floorf(vlo), kv1 = (int)floorf(vhi);
+            const int kvs = max(kv0, 0), kve = min(kv1, nv - 1);
+            if (kvs > kve) continue;
+            const float inv_v = 1.0f / (vhi - vlo);
+            float sum = 0.0f;
+            for (int kv = kvs; kv <= kve; ++kv) {
+                const float wv = dd_weight(kv, kv0, kv1, vlo, vhi, inv_v, zero_tol);
+                const float* row = qv + (size_t)kv * nu;
+                float rs = 0.0f;
+                for (int ku = kus; ku <= kue; ++ku) {
+                    const float wu = dd_weight(ku, ku0, ku1, ulo, uhi, inv_u, zero_tol);
+                    rs = fmaf(row[ku], wu, rs);
+                }
+                sum = fmaf(rs, wv, sum);
+            }
+            acc[i] = fmaf(sum, W, acc[i]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < kBpZChunk; ++i) {
+        const int iz = iz0 + i;
+        if (iz >= nz) break;
+        const size_t idx = ((size_t)iz * ny + iy) * nx + ix;
+        vol[idx] = accumulate ? vol[idx] + acc[i] : acc[i];
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// FDK filter
+// ------------------------------------------------------------------------------------------
+
+constexpr int kMaxLog2N = 13;  // FFT length <= 8192 (nu <= 4097)
+
+int log2_exact(int n) {
+    int l = 0;
+    while ((1 << l) < n) ++l;
+    return ((1 << l) == n) ? l : -1;
+}
+
+// Filter magnitude of Reconstruction.Filter (python/Reconstruction_pycuda.py:186-221) in FFT
+// order, FP64.  The reference takes |fftshift(fft(h))| of the discrete Ram-Lak taps; h is
+// even, so |DFT|(w) = |h0 + 2 sum_{odd n} h(n) cos(n w)| with w = 2 pi f / N (signed f).
+// The cosine table lives in LDS; arguments are reduced exactly as (f*n mod N).
+__global__ void __launch_bounds__(256) spectrum_kernel(double* __restrict__ out, int N,
+                                                       double tau, int ftype, double cutoff) {
+    extern __shared__ double costab[];
+    for (int m = threadIdx.x; m < N; m += blockDim.x) costab[m] = cos(2.0 * kPi * m / N);
+    __syncthreads();
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= N) return;
+    const int fs = (f < N / 2) ? f : f - N;  // signed frequency; f = N/2 is -N/2 as fftshift has it
+    const double h0 = 1.0 / (8.0 * tau * tau);
+    double acc = 0.0;
+    const int fm = ((fs % N) + N) % N;
+    for (int n = 1; n < N / 2; n += 2) {
+        const double hn = -0.5 / ((kPi * tau * n) * (kPi * tau * n));
+        acc += hn * costab[(int)(((long long)fm * n) % N)];
+    }
+    double H = fabs(h0 + 2.0 * acc);
+    const double w = 2.0 * kPi * fs / N;
+    switch (ftype) {
+        case CTR_FILTER_SHEPP_LOGAN: {
+            const double a = w / (2.0 * cutoff);
+            H = (fs == 0) ? 0.0 : H * sin(a) / a;  // DC: H * sin(0) = 0, as the reference
+            break;
+        }
+        case CTR_FILTER_COSINE: H *= cos(w / (2.0 * cutoff)); break;
+        case CTR_FILTER_HAMMING: H *= 0.54 + 0.46 * cos(w / cutoff); break;
+        case CTR_FILTER_HANN: H *= 0.5 + 0.5 * cos(w / cutoff); break;
+        default: break;
+    }
+    if (fabs(w) > kPi * cutoff / (2.0 * tau)) H = 0.0;
+    out[f] = H;
+}
+
+__global__ void twiddle_kernel(float2* __restrict__ tw, int N) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    double s, c;
+    sincos(-2.0 * kPi * k / N, &s, &c);
+    tw[k] = make_float2((float)c, (float)s);
+}
+
+__device__ inline float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+
+constexpr int kFftThreads = 256;
+
+// In-LDS Stockham FFT of length 2^LOG2N (one radix-2 stage when LOG2N is odd, then radix-4
+// stages).  `inv` selects the conjugate twiddles (unnormalised inverse).  Each stage reads
+// all inputs into registers, synchronises, then writes, so one LDS buffer suffices.
+template <int LOG2N>
+__device__ void fft_lds(float2* buf, const float2* __restrict__ tw, bool inv) {
+    constexpr int N = 1 << LOG2N;
+    const int tid = threadIdx.x;
+    int ns = 1;
+    if constexpr (LOG2N & 1) {
+        constexpr int NB = (N / 2 + kFftThreads - 1) / kFftThreads;
+        float2 v[NB][2];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const int j = tid + b * kFftThreads;
+            if (j < N / 2) {
+                v[b][0] = buf[j];
+                v[b][1] = buf[j + N / 2];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const int j = tid + b * kFftThreads;
+            if (j < N / 2) {
+                buf[2 * j] = cadd(v[b][0], v[b][1]);
+                buf[2 * j + 1] = csub(v[b][0], v[b][1]);
+            }
+        }
+        __syncthreads();
+        ns = 2;
+    }
+    constexpr int NB = (N / 4 + kFftThreads - 1) / kFftThreads;
+    for (; ns < N; ns *= 4) {
+        float2 v[NB][4];
+        const int tstep = N / (ns * 4);
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const int j = tid + b * kFftThreads;
+            if (j < N / 4) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) v[b][r] = buf[j + r * (N / 4)];
+                const int kt = (j % ns) * tstep;
+                if (kt != 0) {
+#pragma unroll
+                    for (int r = 1; r < 4; ++r) {
+                        float2 w = tw[r * kt];
+                        if (inv) w.y = -w.y;
+                        v[b][r] = cmul(v[b][r], w);
+                    }
+                }
+                const float2 a0 = cadd(v[b][0], v[b][2]);
+                const float2 a1 = csub(v[b][0], v[b][2]);
+                const float2 a2 = cadd(v[b][1], v[b][3]);
+                const float2 d = csub(v[b][1], v[b][3]);
+                // forward: a3 = -i * d ; inverse: a3 = +i * d
+                const float2 a3 = inv ? make_float2(-d.y, d.x) : make_float2(d.y, -d.x);
+                v[b][0] = cadd(a0, a2);
+                v[b][1] = cadd(a1, a3);
+                v[b][2] = csub(a0, a2);
+                v[b][3] = csub(a1, a3);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const int j = tid + b * kFftThreads;
+            if (j < N / 4) {
+                const int base = (j / ns) * ns * 4 + (j % ns);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) buf[base + r * ns] = v[b][r];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Filtering() / filter_proj (python/Reconstruction_pycuda.py:223-251): two detector rows per
+// complex FFT (row A real part, row B imaginary part).  The spectrum is real and even, so
+// ifft(H * fft(a + i b)) = (a*g) + i (b*g) and the two rows separate without extra work.
+// The cosine pre-weight R/sqrt(R^2 + kk^2 + pp^2) is evaluated at the reference's cell-edge
+// coordinates kk = ((j - nu/2) du + off_u) R/SDD, pp = ((nv/2 - k) dv + off_v) R/SDD.
+template <int LOG2N>
+__global__ void __launch_bounds__(kFftThreads) fdk_filter_kernel(
+    const float* __restrict__ in, float* __restrict__ out, const double* __restrict__ spec,
+    const float2* __restrict__ tw, int n_rows, int nu, int nv, float R, float su, float ou,
+    float sv, float ov, float out_scale) {
+    constexpr int N = 1 << LOG2N;
+    __shared__ float2 buf[N];
+    const int rowA = 2 * blockIdx.x;
+    const int rowB = rowA + 1;
+    const bool hasB = rowB < n_rows;
+    const float pa = ((float)(nv / 2.0f) - (float)(rowA % nv)) * sv + ov;
+    const float pb = ((float)(nv / 2.0f) - (float)(rowB % nv)) * sv + ov;
+    for (int n = threadIdx.x; n < N; n += kFftThreads) {
+        float2 val = make_float2(0.0f, 0.0f);
+        if (n < nu) {
+            const float kk = ((float)n - (float)nu / 2.0f) * su + ou;
+            const float wa = R * rsqrtf(R * R + kk * kk + pa * pa);
+            val.x = in[(size_t)rowA * nu + n] * wa;
+            if (hasB) {
+                const float wb = R * rsqrtf(R * R + kk * kk + pb * pb);
+                val.y = in[(size_t)rowB * nu + n] * wb;
+            }
+        }
+        buf[n] = val;
+    }
+    __syncthreads();
+    fft_lds<LOG2N>(buf, tw, false);
+    for (int n = threadIdx.x; n < N; n += kFftThreads) {
+        const float h = (float)spec[n];
+        buf[n] = make_float2(buf[n].x * h, buf[n].y * h);
+    }
+    __syncthreads();
+    fft_lds<LOG2N>(buf, tw, true);
+    for (int n = threadIdx.x; n < nu; n += kFftThreads) {
+        const float2 val = buf[n];
+        out[(size_t)rowA * nu + n] = val.x * out_scale;
+        if (hasB) out[(size_t)rowB * nu + n] = val.y * out_scale;
+    }
+}
+
+template <int L>
+hipError_t launch_filter(int l2, dim3 grid, hipStream_t st, const float* in, float* out,
+                         const double* spec, const float2* tw, int n_rows, int nu, int nv, float R,
+                         float su, float ou, float sv, float ov, float scale) {
+    if (l2 == L) {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(fdk_filter_kernel<L>), grid, dim3(kFftThreads), 0, st,
+                           in, out, spec, tw, n_rows, nu, nv, R, su, ou, sv, ov, scale);
+        return hipGetLastError();
+    }
+    if constexpr (L < kMaxLog2N) {
+        return launch_filter<L + 1>(l2, grid, st, in, out, spec, tw, n_rows, nu, nv, R, su, ou, sv,
+                                    ov, scale);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+// ==========================================================================================
+// C ABI
+// ==========================================================================================
+
+extern "C" {
+
+int ctr_version(void) { return CTR_ABI_VERSION; }
+
+const char* ctr_strerror(int code) {
+    switch (code) {
+        case CTR_OK: return "ok";
+        case CTR_ERR_INVALID: return "invalid argument (geometry, pointer or size out of range)";
+        case CTR_ERR_UNSUPPORTED: return "unsupported configuration (z-slab branch, detector shape or FFT length)";
+        case CTR_ERR_WORKSPACE: return "workspace too small";
+        case CTR_ERR_HIP: return "HIP runtime error";
+        default: return "unknown error";
+    }
+}
+
+int ctr_select_branches(const ctr_geometry* g, const double* angles, int32_t* branch_out) {
+    int st = check_geometry(g);
+    if (st) return st;
+    if (!angles || !branch_out) return CTR_ERR_INVALID;
+    int rc = CTR_OK;
+    for (int i = 0; i < g->n_views; ++i) {
+        const double th = angles[i];
+        const double z = g->helical * th / (2.0 * kPi);
+        const double S[3] = {-g->sad * std::sin(th), g->sad * std::cos(th), g->source_init[2] + z};
+        const double D[3] = {(g->sdd - g->sad) * std::sin(th), -(g->sdd - g->sad) * std::cos(th),
+                             g->detector_init[2] + z};
+        const double ax = std::fabs(S[0] - D[0]), ay = std::fabs(S[1] - D[1]),
+                     az = std::fabs(S[2] - D[2]);
+        if (ax >= ay && ax >= az) {
+            branch_out[i] = CTR_BRANCH_X;
+        } else if (ay >= ax && ay >= az) {
+            branch_out[i] = CTR_BRANCH_Y;
+        } else {
+            branch_out[i] = CTR_BRANCH_Z;
+            rc = CTR_ERR_UNSUPPORTED;
+        }
+    }
+    return rc;
+}
+
+int ctr_zero_padded_length(int nu) {
+    if (nu < 2) return -1;
+    const int target = 2 * (nu - 1);
+    int n = 1;
+    while (n < target) n <<= 1;
+    return n;
+}
+
+size_t ctr_dd_forward_workspace_bytes(const ctr_geometry* g) {
+    if (check_geometry(g)) return 0;
+    return fp_layout(g).total;
+}
+
+int ctr_dd_forward(const ctr_geometry* g, const double* angles, const int32_t* branch,
+                   const float* vol, float* proj, void* workspace, size_t workspace_bytes,
+                   void* stream) {
+    int st = check_geometry(g);
+    if (st) return st;
+    if (!angles || !branch || !vol || !proj || !workspace) return CTR_ERR_INVALID;
+    const FpLayout L = fp_layout(g);
+    if (workspace_bytes < L.total) return CTR_ERR_WORKSPACE;
+    hipStream_t s = (hipStream_t)stream;
+    char* ws = (char*)workspace;
+    float* tabs = (float*)(ws + L.tabs_off);
+    float* rows = (float*)(ws + L.rows_off);
+    float* vol_t = (float*)(ws + L.volt_off);
+    const int stride = fp_view_stride(g->nu);
+    const int gx = (std::max(g->nu + 1, g->nv + 1) + 255) / 256;
+    hipLaunchKernelGGL(fp_geometry_kernel, dim3(gx, g->n_views), dim3(256), 0, s, *g, angles, branch,
+                       tabs, rows, stride);
+    if ((st = status_from_hip(hipGetLastError()))) return st;
+    hipLaunchKernelGGL(transpose_xy_kernel, dim3((g->nx + 31) / 32, (g->ny + 31) / 32, g->nz),
+                       dim3(32, 8), 0, s, vol, vol_t, g->nx, g->ny);
+    if ((st = status_from_hip(hipGetLastError()))) return st;
+    const dim3 block(64, 4);
+    const dim3 grid((g->nu + 63) / 64, (g->nv + 3) / 4, g->n_views);
+    hipLaunchKernelGGL(dd_forward_kernel, grid, block, 0, s, vol, vol_t, proj, tabs, rows, branch,
+                       g->nx, g->ny, g->nz, g->nu, g->nv, stride, 1e-6f);
+    return status_from_hip(hipGetLastError());
+}
+
+size_t ctr_fdk_filter_workspace_bytes(const ctr_geometry* g) {
+    if (check_geometry(g)) return 0;
+    const int N = ctr_zero_padded_length(g->nu);
+    if (N <= 0) return 0;
+    return align_up((size_t)N * sizeof(float2), 256);
+}
+
+int ctr_filter_spectrum(const ctr_geometry* g, int filter_type, double cutoff, double* spectrum,
+                        void* stream) {
+    int st = check_geometry(g);
+    if (st) return st;
+    if (!spectrum) return CTR_ERR_INVALID;
+    if (filter_type < CTR_FILTER_RAM_LAK || filter_type > CTR_FILTER_HANN) return CTR_ERR_INVALID;
+    if (filter_type != CTR_FILTER_RAM_LAK && !(cutoff > 0)) return CTR_ERR_INVALID;
+    const int N = ctr_zero_padded_length(g->nu);
+    if (N <= 0) return CTR_ERR_INVALID;
+    if (N > (1 << kMaxLog2N)) return CTR_ERR_UNSUPPORTED;
+    const double tau = g->du * g->sad / g->sdd;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(spectrum_kernel, dim3((N + 255) / 256), dim3(256), N * sizeof(double), s,
+                       spectrum, N, tau, filter_type, cutoff);
+    return status_from_hip(hipGetLastError());
+}
+
+int ctr_fdk_filter(const ctr_geometry* g, const double* spectrum, const float* proj_in,
+                   float* proj_out, int n_views, void* workspace, size_t workspace_bytes,
+                   void* stream) {
+    int st = check_geometry(g);
+    if (st) return st;
+    if (!spectrum || !proj_in || !proj_out || !workspace || n_views <= 0) return CTR_ERR_INVALID;
+    if ((double)n_views * g->nv * g->nu >= 2147483647.0) return CTR_ERR_INVALID;
+    const int N = ctr_zero_padded_length(g->nu);
+    if (N <= 0) return CTR_ERR_INVALID;
+    const int l2 = log2_exact(N);
+    if (l2 < 3) return CTR_ERR_UNSUPPORTED;  // nu >= 5
+    if (l2 > kMaxLog2N) return CTR_ERR_UNSUPPORTED;
+    if (workspace_bytes < ctr_fdk_filter_workspace_bytes(g)) return CTR_ERR_WORKSPACE;
+    hipStream_t s = (hipStream_t)stream;
+    float2* tw = (float2*)workspace;
+    hipLaunchKernelGGL(twiddle_kernel, dim3((N + 255) / 256), dim3(256), 0, s, tw, N);
+    if ((st = status_from_hip(hipGetLastError()))) return st;
+    const int n_rows = n_views * g->nv;
+    const double R = g->sad;
+    const double mag = g->sad / g->sdd;  // R / (R + D)
+    const double tau = g->du * mag;
+    const dim3 grid((n_rows + 1) / 2);
+    return status_from_hip(launch_filter<3>(
+        l2, grid, s, proj_in, proj_out, spectrum, tw, n_rows, g->nu, g->nv, (float)R,
+        (float)(g->du * mag), (float)(g->det_offset[0] * mag), (float)(g->dv * mag),
+        (float)(g->det_offset[1] * mag), (float)(tau / N)));
+}
+
+size_t ctr_dd_backproject_workspace_bytes(const ctr_geometry* g, int mode) {
+    if (check_geometry(g)) return 0;
+    (void)mode;
+    return align_up((size_t)g->n_views * kBpViewStride * sizeof(float), 256);
+}
+
+int ctr_dd_backproject(const ctr_geometry* g, const double* angles, const int32_t* branch,
+                       const float* proj, float* vol, int mode, float scale, float zero_tol,
+                       int accumulate, void* workspace, size_t workspace_bytes, void* stream) {
+    (void)branch;
+    int st = check_geometry(g);
+    if (st) return st;
+    if (!angles || !proj || !vol || !workspace) return CTR_ERR_INVALID;
+    if (mode != CTR_BP_FDK_REF && mode != CTR_BP_FDK_EDGE) return CTR_ERR_UNSUPPORTED;
+    if (workspace_bytes < ctr_dd_backproject_workspace_bytes(g, mode)) return CTR_ERR_WORKSPACE;
+    hipStream_t s = (hipStream_t)stream;
+    float* vt = (float*)workspace;
+    hipLaunchKernelGGL(bp_geometry_kernel, dim3((g->n_views + 255) / 256), dim3(256), 0, s, *g,
+                       angles, vt, mode);
+    if ((st = status_from_hip(hipGetLastError()))) return st;
+    // voxel centres: ReconCenter + (i - (n-1)/2) * d with d = (dx, -dy, -dz)
+    const float x0 = (float)(g->recon_center[0] - (g->nx - 1) / 2.0 * g->dx);
+    const float y0 = (float)(g->recon_center[1] + (g->ny - 1) / 2.0 * g->dy);
+    const float z0 = (float)(g->recon_center[2] + (g->nz - 1) / 2.0 * g->dz);
+    const dim3 block(64, 4);
+    const dim3 grid((g->nx + 63) / 64, (g->ny + 3) / 4, (g->nz + kBpZChunk - 1) / kBpZChunk);
+    hipLaunchKernelGGL(dd_backproject_kernel, grid, block, 0, s, proj, vol, vt, g->nx, g->ny,
+                       g->nz, g->nu, g->nv, g->n_views, x0, y0, z0, (float)g->dx, (float)(-g->dy),
+                       (float)(-g->dz), g->detector_shape == CTR_DETECTOR_CURVED ? 1 : 0, scale,
+                       zero_tol, accumulate);
+    return status_from_hip(hipGetLastError());
+}
+
+}  // extern "C"

@@ -1,0 +1,135 @@
+/*
+ * ctrecon.h -- C ABI of the MI355X (gfx950) distance-driven CT projector pair and FDK filter.
+ *
+ * Every buffer is caller-owned device memory (C-contiguous float32 / float64 / int32);
+ * the library never allocates: scratch memory comes in through a workspace argument whose
+ * size the *_workspace_bytes() queries return.  Work is enqueued on the HIP stream passed
+ * as `stream` (a hipStream_t, NULL = default stream); calls return before the GPU finishes.
+ * Every function returns CTR_OK (0) or a CTR_ERR_* code; ctr_strerror() names it.  Invalid
+ * arguments are rejected on the host before anything is launched.
+ *
+ * Array layouts are the reference's (leehoy/CTReconstruction, python/Reconstruction_pycuda.py):
+ *   volume       float32 [nz][ny][nx]            (LoadRecon, :104-105)
+ *   projections  float32 [n_views][nv][nu]       (LoadProj,  :101-102)
+ *   angles       float64 [n_views]               (linspace(StartAngle, EndAngle, n+1)[:-1], :788-789)
+ *   branch       int32   [n_views]               (slab axis per view, :852-853 / :921-922)
+ */
+#ifndef CTRECON_H
+#define CTRECON_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CTR_ABI_VERSION 1
+
+enum ctr_status {
+    CTR_OK = 0,
+    CTR_ERR_INVALID = 1,      /* bad geometry, null pointer, size out of range            */
+    CTR_ERR_UNSUPPORTED = 2,  /* z-slab branch, unknown detector shape, FFT length > 8192 */
+    CTR_ERR_WORKSPACE = 3,    /* workspace smaller than the *_workspace_bytes() query     */
+    CTR_ERR_HIP = 4           /* a HIP launch or runtime call failed                       */
+};
+
+enum ctr_detector_shape { CTR_DETECTOR_FLAT = 0, CTR_DETECTOR_CURVED = 1 };
+
+enum ctr_branch { CTR_BRANCH_X = 0, CTR_BRANCH_Y = 1, CTR_BRANCH_Z = 2 };
+
+enum ctr_filter_type {
+    CTR_FILTER_RAM_LAK = 0, CTR_FILTER_SHEPP_LOGAN = 1, CTR_FILTER_COSINE = 2,
+    CTR_FILTER_HAMMING = 3, CTR_FILTER_HANN = 4
+};
+
+enum ctr_bp_mode {
+    CTR_BP_FDK_REF = 0,   /* reference grid, half a pixel off the forward edges (:353-354)   */
+    CTR_BP_FDK_EDGE = 1,  /* forward projector's true cell edges (:356-357, commented there) */
+    CTR_BP_ADJOINT = 2    /* exact transpose of ctr_dd_forward (no reference counterpart)    */
+};
+
+/* Geometry of one scan, in the reference's params-dict conventions
+ * (python/Reconstruction_pycuda.py:64-99).  Spacings are positive; the library applies the
+ * reference's sign flips (dy, dz, dv -> negative) itself. */
+typedef struct ctr_geometry {
+    int32_t nx, ny, nz;        /* NumberOfImage                                     */
+    int32_t nu, nv;            /* NumberOfDetectorPixels                            */
+    int32_t n_views;           /* NumberOfViews                                     */
+    int32_t detector_shape;    /* ctr_detector_shape (DetectorShape)                */
+    int32_t reserved0;
+    double dx, dy, dz;         /* ImagePixelSpacing [mm]                            */
+    double du, dv;             /* DetectorPixelSize [mm] (curved: arc length at SDD) */
+    double sad;                /* |SourceInit - RotationOrigin|                     */
+    double sdd;                /* |SourceInit - DetectorInit|                       */
+    double source_init[3];     /* SourceInit (the backprojector uses it unrotated)  */
+    double detector_init[3];   /* DetectorInit                                      */
+    double helical;            /* Pitch * nv * dv * SAD / SDD: z advance per 2*pi   */
+    double recon_center[3];    /* ReconCenter                                       */
+    double det_offset[2];      /* DetectorOffset (u, v)                             */
+} ctr_geometry;
+
+/* Library ABI version (CTR_ABI_VERSION of the build). */
+int ctr_version(void);
+
+/* Static message for a ctr_status code.  Replaces the reference's ErrorDescription
+ * (python/Reconstruction_pycuda.py:43-59), which printed and called sys.exit(). */
+const char* ctr_strerror(int code);
+
+/* Host helper: slab branch of every view from the reference's FP64 comparisons
+ * (python/Reconstruction_pycuda.py:828-830, 852-853, 921-922; ties go to x, then y).
+ * `angles` and `branch_out` are HOST arrays of n_views entries.  Returns
+ * CTR_ERR_UNSUPPORTED if a view selects the z-slab branch. */
+int ctr_select_branches(const ctr_geometry* g, const double* angles, int32_t* branch_out);
+
+/* Zero-padded FFT length 2^ceil(log2(2(nu-1))) (python/Reconstruction_pycuda.py:236). */
+int ctr_zero_padded_length(int nu);
+
+/* ---- DD forward projection --------------------------------------------------------------
+ * Replaces Reconstruction.distance_forward's per-view host geometry and H2D copies
+ * (python/Reconstruction_pycuda.py:762-1059) and the kernels distance_project_on_{x,y}3
+ * (python/GPUFuncs_pycuda.py:398-524).  Per-view footprint tables are built on the GPU in
+ * FP64; the projector is a gather (one work-item per detector cell, slabs looped in
+ * registers, no atomics, deterministic).  `proj` is overwritten.  `vol` and `proj` must not
+ * overlap the workspace. */
+size_t ctr_dd_forward_workspace_bytes(const ctr_geometry* g);
+int ctr_dd_forward(const ctr_geometry* g, const double* angles, const int32_t* branch,
+                   const float* vol, float* proj, void* workspace, size_t workspace_bytes,
+                   void* stream);
+
+/* ---- FDK filter -------------------------------------------------------------------------
+ * ctr_filter_spectrum replaces Reconstruction.Filter (python/Reconstruction_pycuda.py:186-221):
+ * writes the filter magnitude for an FFT of length N = ctr_zero_padded_length(nu) in FFT
+ * (ifftshift-ed) order, float64[N], computed in FP64 on the GPU from the cosine series of
+ * the discrete Ram-Lak taps, with the reference's windows, its Shepp-Logan DC zero and its
+ * frequency mask |w| > pi*cutoff/(2*tau), tau = du*SAD/SDD.
+ *
+ * ctr_fdk_filter replaces Filtering()/filter_proj (python/Reconstruction_pycuda.py:223-251):
+ * cosine pre-weight at cell-edge coordinates, zero-padded FFT convolution with `spectrum`,
+ * crop to nu, times tau.  `proj_in` may equal `proj_out` (in place). */
+size_t ctr_fdk_filter_workspace_bytes(const ctr_geometry* g);
+int ctr_filter_spectrum(const ctr_geometry* g, int filter_type, double cutoff,
+                        double* spectrum, void* stream);
+int ctr_fdk_filter(const ctr_geometry* g, const double* spectrum, const float* proj_in,
+                   float* proj_out, int n_views, void* workspace, size_t workspace_bytes,
+                   void* stream);
+
+/* ---- DD backprojection ------------------------------------------------------------------
+ * Replaces Reconstruction.distance_backproj (python/Reconstruction_pycuda.py:320-435) and
+ * the kernel flat_distance_backproj_arb2 (python/GPUFuncs_pycuda.py:837-956), or for
+ * curved detectors curved_distance_backproj_arb with the numba units
+ * (python/GPUFuncs_numba.py:537-685, python/Reconstruction_numba.py:399-413).
+ * Voxel-driven gather over all views in one launch, no atomics.
+ * vol = (accumulate ? vol : 0) + sum_views scale * BP_view(proj); the reference's FDK uses
+ * scale = dtheta (Q = proj * dtheta, :411).  Overlap weights below `zero_tol` are zeroed
+ * (0 = never, as the pycuda kernel; 1e-5 = the numba kernels). */
+size_t ctr_dd_backproject_workspace_bytes(const ctr_geometry* g, int mode);
+int ctr_dd_backproject(const ctr_geometry* g, const double* angles, const int32_t* branch,
+                       const float* proj, float* vol, int mode, float scale, float zero_tol,
+                       int accumulate, void* workspace, size_t workspace_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CTRECON_H */
