@@ -1,0 +1,219 @@
+"""Benchmark: FDK reconstruction of the full-size cone-beam scan on one MI355X.
+
+Workload (default ``--config c3``): 512^3 volume (0.25 mm voxels) of 40 seeded random
+ellipsoids (seed 3) times mu_water = 0.02/mm, 720 views over 360 degrees, 1024 x 768 flat
+detector at 0.4 mm, SID 1000 mm, SDD 1500 mm, Shepp-Logan window; projections from the DD
+forward projector plus Poisson noise at 1e5 photons.  All inputs are resident in HBM before
+timing.  One step = one FDK reconstruction: cosine weight + ramp filter of every detector
+row, then the FDK-weighted DD backprojection of all 720 views into the 512^3 volume.  The
+forward projection is timed separately and reported beside the step.
+
+Prints one JSON line.  ``value`` is seconds per FDK reconstruction (lower is better);
+``roofline`` prices the dominant kernel (the backprojection) against the 8 TB/s HBM peak
+with algorithmic bytes 4*(views*nv*nu + nx*ny*nz) per launch; ``cpu_baseline`` times the FP64
+NumPy oracle on a bounded sample (2 views, central 8-slice slab) and scales it to a full
+reconstruction.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+CONFIGS = {
+    "c3": dict(n=512, voxel=0.25, views=720, nu=1024, nv=768, pitch=0.4, ellipsoids=40, seed=3,
+               noise=True, workload="cone-beam FDK 512^3, 720 views, 1024x768 detector"),
+    "c2": dict(n=256, voxel=0.5, views=360, nu=384, nv=384, pitch=1.0, ellipsoids=20, seed=2,
+               noise=False, workload="cone-beam FDK 256^3, 360 views, 384x384 detector"),
+}
+
+
+def make_params(cfg, filter_type="shepp-logan"):
+    n = cfg["n"]
+    return {
+        "SourceInit": [0, 1000.0, 0], "DetectorInit": [0, -500.0, 0], "StartAngle": 0,
+        "EndAngle": 2 * np.pi, "NumberOfDetectorPixels": [cfg["nu"], cfg["nv"]],
+        "DetectorPixelSize": [cfg["pitch"], cfg["pitch"]], "NumberOfViews": cfg["views"],
+        "ImagePixelSpacing": [cfg["voxel"]] * 3, "NumberOfImage": [n, n, n],
+        "PhantomCenter": [0, 0, 0], "RotationOrigin": [0, 0, 0], "ReconCenter": [0, 0, 0],
+        "Method": "Distance", "FilterType": filter_type, "cutoff": 1, "GPU": 1,
+        "DetectorShape": "Flat", "Pitch": 0, "DetectorOffset": [0, 0],
+    }
+
+
+def traffic_from_profile(name):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled by the
+    copy-kernel calibration), recorded in profiles/<name>.json; None when absent."""
+    path = os.path.join(ROOT, "profiles", name + ".json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("traffic_bytes")
+
+
+def cpu_baseline(params, q_gpu, geom, dev, sample_views=(0, 1), slab=8):
+    """Time the FP64 NumPy oracle (filter + FDK backprojection) on a bounded sample and scale
+    it to a full reconstruction; also return the sample's error against the GPU result."""
+    import torch
+
+    import ctreconstruction_amd as ctr
+    from oracle import dd_oracle as O
+
+    g = O.Geometry(params)
+    views = list(sample_views)
+    # filter: time the oracle on the sample views
+    proj_s = q_gpu["raw"][views].cpu().numpy()
+    t0 = time.perf_counter()
+    q_ref = O.filter_projections(g, proj_s)
+    t_filter = (time.perf_counter() - t0) / len(views)
+    # backprojection: central slab of `slab` slices, sample views
+    p_slab = dict(params)
+    p_slab["NumberOfImage"] = [g.nx, g.ny, slab]
+    gs = O.Geometry(p_slab)
+    t0 = time.perf_counter()
+    vol_ref = np.zeros((slab, g.ny, g.nx))
+    for n, i in enumerate(views):
+        vol_ref += O.backproject_view(gs, q_ref[n].astype(np.float64) * g.dtheta, i, "ref")
+    t_bp = (time.perf_counter() - t0) / len(views)
+    # the same sample on the GPU: filter of the sample views, backprojection into the slab
+    geom_slab = ctr.ScanGeometry(p_slab).with_views(views)
+    q_s = ctr.fdk_filter(q_gpu["raw"][views].contiguous(), geom)
+    vol_gpu = ctr.dd_backproject(q_s, geom_slab).cpu().numpy()
+    err = float(np.abs(vol_gpu - vol_ref).max() / np.abs(vol_ref).max())
+    err_filter = float(np.abs(q_s.cpu().numpy() - q_ref).max() / np.abs(q_ref).max())
+    full = t_filter * g.nview + t_bp * g.nview * (g.nz / slab)
+    return {
+        "value": full, "unit": "s", "cores": 1, "kind": "port",
+        "sample": (f"FP64 NumPy oracle, single thread: filter of {len(views)} views and FDK "
+                   f"backprojection of {len(views)} views into the central {slab}-slice slab, "
+                   f"scaled linearly to {g.nview} views x {g.nz} slices "
+                   f"(filter {t_filter:.3f} s/view, backprojection {t_bp:.3f} s/view/slab)"),
+        "host_cpus": os.cpu_count(), "host_affinity": len(os.sched_getaffinity(0)),
+        "sample_error_vs_oracle": {"fdk_backprojection": err, "filter": err_filter},
+    }
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="also run one device copy of this many MiB (rocprof counter calibration)")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is designed, not "
+                                   "built (see DESIGN.md)", "n_gpus": args.gpus}))
+        return 2
+
+    import torch
+
+    import ctreconstruction_amd as ctr
+    from ctreconstruction_amd import phantom
+
+    cfg = CONFIGS[args.config]
+    params = make_params(cfg)
+    geom = ctr.ScanGeometry(params)
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.current_stream(dev)
+
+    t_setup = time.perf_counter()
+    ell = phantom.random_ellipsoids(cfg["ellipsoids"], phantom.half_extent(geom), cfg["seed"])
+    vol = phantom.ellipsoid_volume(geom, ell, device=dev) * phantom.MU_WATER
+    # forward projection (timed separately from the FDK step)
+    proj = ctr.dd_forward(vol, geom)
+    torch.cuda.synchronize()
+    fp_times = []
+    for _ in range(2):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        ctr.dd_forward(vol, geom, out=proj)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        fp_times.append(e0.elapsed_time(e1) / 1e3)
+    if cfg["noise"]:
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(cfg["seed"])
+        proj = phantom.poisson_noise(proj, 1e5, generator=gen)
+    spectrum = ctr.filter_spectrum(geom, device=dev)
+    q = torch.empty_like(proj)
+    rec = torch.empty(geom.vol_shape, dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t_setup
+
+    def step(ev=None):
+        ctr.fdk_filter(proj, geom, spectrum, out=q)
+        if ev is not None:
+            ev[0].record(stream)
+        ctr.dd_backproject(q, geom, mode="ref", out=rec)
+        if ev is not None:
+            ev[1].record(stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    bp_events = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                 for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(bp_events[k])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    bp_s = float(np.mean([a.elapsed_time(b) for a, b in bp_events])) / 1e3
+    sec = elapsed / args.steps
+
+    if args.calibrate_copy_mib:
+        n = args.calibrate_copy_mib * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+
+    nvox = geom.nx * geom.ny * geom.nz
+    ncell = geom.nview * geom.nv * geom.nu
+    bp_bytes = 4.0 * (ncell + nvox)
+    achieved = bp_bytes / bp_s / 1e9
+    updates = float(nvox) * geom.nview
+    fp_s = float(np.median(fp_times))
+    ray_slab = float(ncell) * geom.ny  # every ray crosses n_slab slabs (ny == nx here)
+    out = {
+        "metric": "FDK 512³/720-view recon s; DD proj GUPS; SART vs learned time-to-RMSE",
+        "value": sec, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": sec * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded random ellipsoids, Poisson noise 1e5 photons)"
+        if cfg["noise"] else "synthetic (seeded random ellipsoids)",
+        "config": {"workload": cfg["workload"], "volume": [geom.nz, geom.ny, geom.nx],
+                   "voxel_mm": cfg["voxel"], "views": geom.nview, "detector": [geom.nv, geom.nu],
+                   "pixel_mm": cfg["pitch"], "sad_mm": geom.sad, "sdd_mm": geom.sdd,
+                   "filter": geom.filter_type, "step": "fdk_filter + dd_backproject (mode ref)"},
+        "roofline": {"bound": "hbm", "kernel": "dd_backproject_kernel", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "algorithmic_bytes": bp_bytes,
+                     "traffic": traffic_from_profile(f"dd_backproject_{args.config}_traffic")},
+        "fdk_backprojection": {"s": bp_s, "gups": updates / bp_s / 1e9},
+        "fdk_filter": {"s": sec - bp_s},
+        "dd_forward": {"s": fp_s, "gups": updates / fp_s / 1e9, "ray_slab_updates_per_s": ray_slab / fp_s},
+        "sart_vs_learned_time_to_rmse": None,
+        "setup_s": t_setup,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(params, {"raw": proj}, geom, dev)
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

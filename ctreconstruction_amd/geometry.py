@@ -132,6 +132,17 @@ class ScanGeometry:
             self._device_cache[key] = (ang, br)
         return self._device_cache[key]
 
+    def with_views(self, views) -> "ScanGeometry":
+        """The same scan restricted to ``views`` (their angles and branches; dtheta of the full scan)."""
+        import copy
+        sub = copy.copy(self)
+        idx = np.asarray(views, dtype=np.int64)
+        sub.angles = self.angles[idx].copy()
+        sub.branch = self.branch[idx].copy()
+        sub.nview = len(idx)
+        sub._device_cache = {}
+        return sub
+
     def check_supported(self):
         if np.any(self.branch == N.CTR_BRANCH_Z):
             raise N.CtrError(N.CTR_ERR_UNSUPPORTED, "z-slab branch (|S_z - D_z| dominates)")

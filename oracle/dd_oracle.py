@@ -484,8 +484,9 @@ def backproject_view(g: Geometry, q: np.ndarray, i: int, mode: str = "ref",
             for sz in (1, -1):
                 vs.append((Sz - (Z + sz * hz)) / (Sy - yr))
     shape = (g.nz, g.ny, g.nx)
-    u_min = np.broadcast_to(np.minimum.reduce(us), shape)
-    u_max = np.broadcast_to(np.maximum.reduce(us), shape)
+    # the u-footprint depends on the voxel column only ([1, ny, nx]); v on the voxel ([nz, ny, nx])
+    u_min = np.minimum.reduce(us)
+    u_max = np.maximum.reduce(us)
     v_min = np.broadcast_to(np.minimum.reduce(vs), shape)
     v_max = np.broadcast_to(np.maximum.reduce(vs), shape)
     if g.shape == "Curved":
@@ -501,22 +502,26 @@ def backproject_view(g: Geometry, q: np.ndarray, i: int, mode: str = "ref",
     iv2 = np.floor((cv2 - p[0]) / dv).astype(np.int64)
     su, eu = np.minimum(iu1, iu2), np.maximum(iu1, iu2)
     sv, ev = np.minimum(iv1, iv2), np.maximum(iv1, iv2)
-    W = np.broadcast_to((R * R) / ((R - y_rot) * (R - y_rot)), shape)
+    W = (R * R) / ((R - y_rot) * (R - y_rot))  # [1, ny, nx]
     acc = np.zeros(shape, dtype=dt)
+    u_taps = []
+    for b in range(int(np.max(eu - su)) + 1):
+        l = su + b
+        ok_l = (l <= eu) & (l >= 0) & (l <= g.nu - 1)
+        if ok_l.any():
+            w2 = _overlap_weights(cu1, cu2, su, eu, l, ki, du, False, zero_tol, dt)
+            u_taps.append((np.clip(l, 0, g.nu - 1), np.where(ok_l, w2 * W, dt(0)).astype(dt)))
     for a in range(int(np.max(ev - sv)) + 1):
         k = sv + a
         ok_k = (k <= ev) & (k >= 0) & (k <= g.nv - 1)
         if not ok_k.any():
             continue
-        w1 = _overlap_weights(cv1, cv2, sv, ev, k, p, dv, True, zero_tol, dt)
-        for b in range(int(np.max(eu - su)) + 1):
-            l = su + b
-            ok = ok_k & (l <= eu) & (l >= 0) & (l <= g.nu - 1)
-            if not ok.any():
-                continue
-            w2 = _overlap_weights(cu1, cu2, su, eu, l, ki, du, False, zero_tol, dt)
-            qv = q[np.clip(k, 0, g.nv - 1), np.clip(l, 0, g.nu - 1)].astype(dt)
-            acc = acc + np.where(ok, (qv * w1 * w2 * W).astype(dt), dt(0))
+        w1 = np.where(ok_k, _overlap_weights(cv1, cv2, sv, ev, k, p, dv, True, zero_tol, dt), dt(0))
+        kc = np.clip(k, 0, g.nv - 1)
+        row_sum = np.zeros(shape, dtype=dt)
+        for lc, w2W in u_taps:
+            row_sum = row_sum + (q[kc, lc].astype(dt) * w2W).astype(dt)
+        acc = acc + (row_sum * w1).astype(dt)
     return acc
 
 
