@@ -419,6 +419,251 @@ __global__ void __launch_bounds__(256) dd_backproject_kernel(
     }
 }
 
+// Tiled FDK backprojection: the same per-voxel semantics as dd_backproject_kernel, organised
+// for gfx950.  A workgroup owns a 16 x 16 x ZC voxel tile; for every view it stages the detector
+// window that the tile's bounding box projects to in LDS (double-buffered, one barrier per
+// view), each work-item computes the u-overlap weights of its voxel column once per view and
+// then streams the window's rows: a row is filtered in u once and distributed to the ZC voxels
+// of the column by their v-overlap.  Tiles are assigned so that each XCD holds contiguous
+// z-chunks, whose windows share detector rows in that XCD's L2.  If a window exceeds the LDS
+// staging buffer, that view is gathered from global memory instead (uniform branch).
+constexpr int kTileX = 16, kTileY = 16;
+constexpr int kWinCap = 2048;          // floats per LDS window buffer
+constexpr int kMaxUTaps = 4;           // u taps held in registers (more are looped)
+constexpr int kPrefetch = kWinCap / 256;
+
+struct BpView {
+    float c, s, Sx, Sy, Sz, Au, Bu, Av, Bv, R;
+};
+
+__device__ inline BpView load_view(const float* __restrict__ vt, int view) {
+    const float* T = vt + (size_t)view * kBpViewStride;
+    BpView v;
+    v.c = T[0]; v.s = T[1]; v.Sx = T[2]; v.Sy = T[3]; v.Sz = T[4];
+    v.Au = T[5]; v.Bu = T[6]; v.Av = T[7]; v.Bv = T[8]; v.R = T[9];
+    return v;
+}
+
+// u index coordinate of the ray through the source with in-plane slope `su`
+__device__ inline float u_index(const BpView& v, float su, int curved) {
+    return curved ? v.Au - v.Bu * atanf(su) : v.Au - v.Bu * su;
+}
+
+// Detector window (first row, rows, first column, columns) covering the projection of the box
+// [xa, xb] x [ya, yb] x [za, zb]: the footprint of a box is the hull of its corners' footprints;
+// one cell of margin absorbs rounding differences against the per-voxel computation.
+__device__ inline void tile_window(const BpView& v, float xa, float xb, float ya, float yb,
+                                   float za, float zb, int curved, int nu, int nv, int& wv0,
+                                   int& wvn, int& wu0, int& wun) {
+    float smin = 3.0e38f, smax = -3.0e38f, rmin = 3.0e38f, rmax = -3.0e38f;
+#pragma unroll
+    for (int corner = 0; corner < 4; ++corner) {
+        const float x = (corner & 1) ? xb : xa;
+        const float y = (corner & 2) ? yb : ya;
+        const float xr = x * v.c + y * v.s;
+        const float yr = y * v.c - x * v.s;
+        const float r = 1.0f / (v.Sy - yr);
+        const float su = (v.Sx - xr) * r;
+        smin = fminf(smin, su);
+        smax = fmaxf(smax, su);
+        rmin = fminf(rmin, r);
+        rmax = fmaxf(rmax, r);
+    }
+    const float f1 = u_index(v, smin, curved), f2 = u_index(v, smax, curved);
+    const int u0 = max((int)floorf(fminf(f1, f2)) - 1, 0);
+    const int u1 = min((int)floorf(fmaxf(f1, f2)) + 1, nu - 1);
+    const float ehi = v.Sz - za, elo = v.Sz - zb;  // za <= zb
+    const float smx = ehi * (ehi >= 0.0f ? rmax : rmin);
+    const float smn = elo * (elo >= 0.0f ? rmin : rmax);
+    const float g1 = v.Av - v.Bv * smn, g2 = v.Av - v.Bv * smx;
+    const int v0 = max((int)floorf(fminf(g1, g2)) - 1, 0);
+    const int v1 = min((int)floorf(fmaxf(g1, g2)) + 1, nv - 1);
+    wu0 = u0;
+    wun = max(u1 - u0 + 1, 0);  // 0: the box projects outside the detector
+    wv0 = v0;
+    wvn = max(v1 - v0 + 1, 0);
+}
+
+// 0: window empty (skip the view), 1: staged in LDS, 2: too large, gather from global memory
+__device__ inline int window_mode(int wvn, int wun) {
+    if (wvn <= 0 || wun <= 0) return 0;
+    return (wvn * wun <= kWinCap) ? 1 : 2;
+}
+
+template <int ZC, bool FROM_LDS>
+__device__ inline void bp_column_view(const BpView& v, const float* __restrict__ src, int stride,
+                                      int row0, int nrows, int col0, int ncols, float X, float Y,
+                                      float Z0, float dz, float hx, float hy, float hz, int nz_left,
+                                      int curved, float scale, float zero_tol, int nu, int nv,
+                                      float (&acc)[ZC]) {
+    const float xr0 = X * v.c + Y * v.s;
+    const float yr0 = Y * v.c - X * v.s;
+    float smin = 3.0e38f, smax = -3.0e38f, rmin = 3.0e38f, rmax = -3.0e38f;
+#pragma unroll
+    for (int corner = 0; corner < 4; ++corner) {
+        const float sx = (corner & 1) ? -hx : hx;
+        const float sy = (corner & 2) ? -hy : hy;
+        const float xr = xr0 + sx * v.c + sy * v.s;
+        const float yr = yr0 + sy * v.c - sx * v.s;
+        const float r = 1.0f / (v.Sy - yr);
+        const float su = (v.Sx - xr) * r;
+        smin = fminf(smin, su);
+        smax = fmaxf(smax, su);
+        rmin = fminf(rmin, r);
+        rmax = fmaxf(rmax, r);
+    }
+    const float fu1 = u_index(v, smin, curved), fu2 = u_index(v, smax, curved);
+    const float ulo = fminf(fu1, fu2), uhi = fmaxf(fu1, fu2);
+    const int ku0 = (int)floorf(ulo), ku1 = (int)floorf(uhi);
+    // taps inside the detector and the source window
+    const int kus = max(max(ku0, 0), col0), kue = min(min(ku1, nu - 1), col0 + ncols - 1);
+    if (kus > kue) return;
+    const float inv_u = 1.0f / (uhi - ulo);
+    float wu[kMaxUTaps];
+#pragma unroll
+    for (int j = 0; j < kMaxUTaps; ++j) {
+        const int ku = kus + j;
+        wu[j] = (ku <= kue) ? dd_weight(ku, ku0, ku1, ulo, uhi, inv_u, zero_tol) : 0.0f;
+    }
+    const float dd = v.R - yr0;
+    const float W = scale * (v.R * v.R) / (dd * dd);
+    // v-footprints of the ZC voxels (index space)
+    float vlo[ZC], vhi[ZC], inv_v[ZC];
+    float rlo = 3.0e38f, rhi = -3.0e38f;
+#pragma unroll
+    for (int i = 0; i < ZC; ++i) {
+        const float e = v.Sz - fmaf((float)i, dz, Z0);
+        const float ep = e + hz, em = e - hz;
+        const float smx = ep * (ep >= 0.0f ? rmax : rmin);
+        const float smn = em * (em >= 0.0f ? rmin : rmax);
+        const float g1 = v.Av - v.Bv * smn, g2 = v.Av - v.Bv * smx;
+        vlo[i] = fminf(g1, g2);
+        vhi[i] = fmaxf(g1, g2);
+        inv_v[i] = 1.0f / (vhi[i] - vlo[i]);
+        if (i < nz_left) {
+            rlo = fminf(rlo, vlo[i]);
+            rhi = fmaxf(rhi, vhi[i]);
+        }
+    }
+    const int ks = max(max((int)floorf(rlo), 0), row0);
+    const int ke = min(min((int)floorf(rhi), nv - 1), row0 + nrows - 1);
+    const int cbase = kus - col0;
+    const int ntaps = kue - kus + 1;
+    for (int k = ks; k <= ke; ++k) {
+        const float* row = src + (size_t)(k - row0) * stride + cbase;
+        float rk = 0.0f;
+#pragma unroll
+        for (int j = 0; j < kMaxUTaps; ++j) rk = fmaf(wu[j], row[min(j, ntaps - 1)], rk);
+        for (int j = kMaxUTaps; j < ntaps; ++j)
+            rk = fmaf(dd_weight(kus + j, ku0, ku1, ulo, uhi, inv_u, zero_tol), row[j], rk);
+        rk *= W;
+        const float fk = (float)k, fk1 = (float)(k + 1);
+#pragma unroll
+        for (int i = 0; i < ZC; ++i) {
+            float w = (fminf(vhi[i], fk1) - fmaxf(vlo[i], fk)) * inv_v[i];
+            w = (w < zero_tol || w < 0.0f) ? 0.0f : w;
+            acc[i] = fmaf(w, rk, acc[i]);
+        }
+    }
+}
+
+template <int ZC>
+__global__ void __launch_bounds__(256) dd_backproject_tile_kernel(
+    const float* __restrict__ q, float* __restrict__ vol, const float* __restrict__ vt,
+    int nx, int ny, int nz, int nu, int nv, int n_views, float x0, float y0, float z0,
+    float dx, float dy, float dz, int curved, float scale, float zero_tol, int accumulate,
+    int tiles_x, int tiles_y, int tiles_z) {
+    __shared__ float win[2][kWinCap];
+    // XCD-aware tile order: workgroup b runs on XCD b % 8; give each XCD a contiguous range
+    const int T = tiles_x * tiles_y * tiles_z;
+    const int b = blockIdx.x;
+    const int xcd = b & 7, local = b >> 3, qd = T >> 3, rm = T & 7;
+    const int t = (xcd < rm) ? xcd * (qd + 1) + local : rm * (qd + 1) + (xcd - rm) * qd + local;
+    const int tx = t % tiles_x;
+    const int ty = (t / tiles_x) % tiles_y;
+    const int tz = t / (tiles_x * tiles_y);
+    const int lx = threadIdx.x & (kTileX - 1), ly = threadIdx.x / kTileX;
+    const int ix = tx * kTileX + lx, iy = ty * kTileY + ly, iz0 = tz * ZC;
+    const bool active = (ix < nx) && (iy < ny);
+    const float hx = 0.5f * fabsf(dx), hy = 0.5f * fabsf(dy), hz = 0.5f * fabsf(dz);
+    const float X = fmaf((float)ix, dx, x0);
+    const float Y = fmaf((float)iy, dy, y0);
+    const float Z0 = fmaf((float)iz0, dz, z0);
+    const int nz_left = min(ZC, nz - iz0);
+    // tile bounding box
+    const float xa0 = fmaf((float)(tx * kTileX), dx, x0);
+    const float xb0 = fmaf((float)min(tx * kTileX + kTileX - 1, nx - 1), dx, x0);
+    const float ya0 = fmaf((float)(ty * kTileY), dy, y0);
+    const float yb0 = fmaf((float)min(ty * kTileY + kTileY - 1, ny - 1), dy, y0);
+    const float za0 = Z0;
+    const float zb0 = fmaf((float)(iz0 + nz_left - 1), dz, z0);
+    const float xa = fminf(xa0, xb0) - hx, xb = fmaxf(xa0, xb0) + hx;
+    const float ya = fminf(ya0, yb0) - hy, yb = fmaxf(ya0, yb0) + hy;
+    const float za = fminf(za0, zb0) - hz, zb = fmaxf(za0, zb0) + hz;
+
+    float acc[ZC];
+#pragma unroll
+    for (int i = 0; i < ZC; ++i) acc[i] = 0.0f;
+
+    int wv0, wvn, wu0, wun;
+    BpView cur = load_view(vt, 0);
+    tile_window(cur, xa, xb, ya, yb, za, zb, curved, nu, nv, wv0, wvn, wu0, wun);
+    float pf[kPrefetch];
+    if (window_mode(wvn, wun) == 1) {
+        for (int e = threadIdx.x; e < wvn * wun; e += 256)
+            win[0][e] = q[(size_t)(wv0 + e / wun) * nu + wu0 + e % wun];
+    }
+    __syncthreads();
+    for (int view = 0; view < n_views; ++view) {
+        const int buf = view & 1;
+        const int mode = window_mode(wvn, wun);
+        // prefetch the next view's window into registers
+        int nv0 = 0, nvn = 0, nu0 = 0, nun = 0;
+        BpView nxt = cur;
+        const bool has_next = view + 1 < n_views;
+        bool nfits = false;
+        if (has_next) {
+            nxt = load_view(vt, view + 1);
+            tile_window(nxt, xa, xb, ya, yb, za, zb, curved, nu, nv, nv0, nvn, nu0, nun);
+            nfits = window_mode(nvn, nun) == 1;
+            if (nfits) {
+                const float* qn = q + (size_t)(view + 1) * nv * nu;
+#pragma unroll
+                for (int p = 0; p < kPrefetch; ++p) {
+                    const int e = threadIdx.x + p * 256;
+                    if (e < nvn * nun) pf[p] = qn[(size_t)(nv0 + e / nun) * nu + nu0 + e % nun];
+                }
+            }
+        }
+        if (active && mode != 0) {
+            if (mode == 1) {
+                bp_column_view<ZC, true>(cur, win[buf], wun, wv0, wvn, wu0, wun, X, Y, Z0, dz, hx, hy,
+                                         hz, nz_left, curved, scale, zero_tol, nu, nv, acc);
+            } else {
+                bp_column_view<ZC, false>(cur, q + (size_t)view * nv * nu, nu, 0, nv, 0, nu, X, Y, Z0,
+                                          dz, hx, hy, hz, nz_left, curved, scale, zero_tol, nu, nv, acc);
+            }
+        }
+        if (has_next && nfits) {
+#pragma unroll
+            for (int p = 0; p < kPrefetch; ++p) {
+                const int e = threadIdx.x + p * 256;
+                if (e < nvn * nun) win[buf ^ 1][e] = pf[p];
+            }
+        }
+        __syncthreads();
+        cur = nxt;
+        wv0 = nv0; wvn = nvn; wu0 = nu0; wun = nun;
+    }
+    if (!active) return;
+#pragma unroll
+    for (int i = 0; i < ZC; ++i) {
+        if (i >= nz_left) break;
+        const size_t idx = ((size_t)(iz0 + i) * ny + iy) * nx + ix;
+        vol[idx] = accumulate ? vol[idx] + acc[i] : acc[i];
+    }
+}
+
 // ------------------------------------------------------------------------------------------
 // FDK filter
 // ------------------------------------------------------------------------------------------
@@ -783,12 +1028,17 @@ int ctr_dd_backproject(const ctr_geometry* g, const double* angles, const int32_
     const float x0 = (float)(g->recon_center[0] - (g->nx - 1) / 2.0 * g->dx);
     const float y0 = (float)(g->recon_center[1] + (g->ny - 1) / 2.0 * g->dy);
     const float z0 = (float)(g->recon_center[2] + (g->nz - 1) / 2.0 * g->dz);
-    const dim3 block(64, 4);
-    const dim3 grid((g->nx + 63) / 64, (g->ny + 3) / 4, (g->nz + kBpZChunk - 1) / kBpZChunk);
-    hipLaunchKernelGGL(dd_backproject_kernel, grid, block, 0, s, proj, vol, vt, g->nx, g->ny,
-                       g->nz, g->nu, g->nv, g->n_views, x0, y0, z0, (float)g->dx, (float)(-g->dy),
-                       (float)(-g->dz), g->detector_shape == CTR_DETECTOR_CURVED ? 1 : 0, scale,
-                       zero_tol, accumulate);
+    constexpr int ZC = 8;
+    const int tiles_x = (g->nx + kTileX - 1) / kTileX;
+    const int tiles_y = (g->ny + kTileY - 1) / kTileY;
+    const int tiles_z = (g->nz + ZC - 1) / ZC;
+    const long long tiles = (long long)tiles_x * tiles_y * tiles_z;
+    if (tiles >= 2147483647LL) return CTR_ERR_INVALID;
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(dd_backproject_tile_kernel<ZC>), dim3((unsigned)tiles), dim3(256),
+                       0, s, proj, vol, vt, g->nx, g->ny, g->nz, g->nu, g->nv, g->n_views, x0, y0, z0,
+                       (float)g->dx, (float)(-g->dy), (float)(-g->dz),
+                       g->detector_shape == CTR_DETECTOR_CURVED ? 1 : 0, scale, zero_tol, accumulate,
+                       tiles_x, tiles_y, tiles_z);
     return status_from_hip(hipGetLastError());
 }
 

@@ -30,6 +30,11 @@ CASES = {
                     Pitch=0.5, EndAngle=4 * np.pi),
     "curved": dict(NumberOfImage=[24, 24, 16], NumberOfDetectorPixels=[48, 30], NumberOfViews=12,
                    DetectorShape="Curved"),
+    # 2 mm voxels seen by 0.2 mm pixels: more u taps than are held in registers, detector windows
+    # larger than the LDS staging buffer, and voxels projecting outside the detector
+    "wide_footprint": dict(NumberOfImage=[20, 20, 10], ImagePixelSpacing=[2.0, 2.0, 2.0],
+                           NumberOfDetectorPixels=[300, 150], DetectorPixelSize=[0.2, 0.2], NumberOfViews=6,
+                           StartAngle=0.1, EndAngle=0.1 + 2 * np.pi),
     "curved_fan_2d": dict(NumberOfImage=[48, 48, 1], ImagePixelSpacing=[0.5, 0.5, 0.5],
                           NumberOfDetectorPixels=[96, 1], DetectorPixelSize=[0.5, 0.75], NumberOfViews=24,
                           DetectorShape="Curved"),
