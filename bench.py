@@ -199,7 +199,7 @@ def main(argv=None):
                    "voxel_mm": cfg["voxel"], "views": geom.nview, "detector": [geom.nv, geom.nu],
                    "pixel_mm": cfg["pitch"], "sad_mm": geom.sad, "sdd_mm": geom.sdd,
                    "filter": geom.filter_type, "step": "fdk_filter + dd_backproject (mode ref)"},
-        "roofline": {"bound": "hbm", "kernel": "dd_backproject_kernel", "achieved": achieved,
+        "roofline": {"bound": "hbm", "kernel": "dd_backproject_tile_kernel<8>", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "algorithmic_bytes": bp_bytes,
                      "traffic": traffic_from_profile(f"dd_backproject_{args.config}_traffic")},

@@ -422,15 +422,20 @@ __global__ void __launch_bounds__(256) dd_backproject_kernel(
 // Tiled FDK backprojection: the same per-voxel semantics as dd_backproject_kernel, organised
 // for gfx950.  A workgroup owns a 16 x 16 x ZC voxel tile; for every view it stages the detector
 // window that the tile's bounding box projects to in LDS (double-buffered, one barrier per
-// view), each work-item computes the u-overlap weights of its voxel column once per view and
-// then streams the window's rows: a row is filtered in u once and distributed to the ZC voxels
-// of the column by their v-overlap.  Tiles are assigned so that each XCD holds contiguous
-// z-chunks, whose windows share detector rows in that XCD's L2.  If a window exceeds the LDS
-// staging buffer, that view is gathered from global memory instead (uniform branch).
+// view).  Each work-item computes the u-overlap weights of its voxel column once per view,
+// filters every window row it needs in u once, and keeps the running sum of those rows (a row
+// integral C(v), piecewise linear in the index-space coordinate v) in its own LDS column.  The
+// v-overlap-weighted sum of rows for a voxel whose footprint is [v_lo, v_hi] is then
+// (C(v_hi) - C(v_lo)) / (v_hi - v_lo): the reference's weights are exactly the overlap lengths
+// of the footprint with the rows, divided by its width.  Weight zeroing (numba variant) and
+// chunks spanning more than kMaxRows rows take the explicit per-row path instead.  Tiles are
+// assigned so that each XCD holds contiguous z-chunks, whose windows share detector rows in
+// that XCD's L2.  A window larger than the LDS buffer is gathered from global memory.
 constexpr int kTileX = 16, kTileY = 16;
-constexpr int kWinCap = 2048;          // floats per LDS window buffer
+constexpr int kWinCap = 1024;          // floats per LDS window buffer
 constexpr int kMaxUTaps = 4;           // u taps held in registers (more are looped)
 constexpr int kPrefetch = kWinCap / 256;
+constexpr int kMaxRows = 16;           // rows per column chunk on the row-integral path
 
 struct BpView {
     float c, s, Sx, Sy, Sz, Au, Bu, Av, Bv, R;
@@ -495,7 +500,7 @@ __device__ inline void bp_column_view(const BpView& v, const float* __restrict__
                                       int row0, int nrows, int col0, int ncols, float X, float Y,
                                       float Z0, float dz, float hx, float hy, float hz, int nz_left,
                                       int curved, float scale, float zero_tol, int nu, int nv,
-                                      float (&acc)[ZC]) {
+                                      float* __restrict__ scan, float (&acc)[ZC]) {
     const float xr0 = X * v.c + Y * v.s;
     const float yr0 = Y * v.c - X * v.s;
     float smin = 3.0e38f, smax = -3.0e38f, rmin = 3.0e38f, rmax = -3.0e38f;
@@ -547,8 +552,45 @@ __device__ inline void bp_column_view(const BpView& v, const float* __restrict__
     }
     const int ks = max(max((int)floorf(rlo), 0), row0);
     const int ke = min(min((int)floorf(rhi), nv - 1), row0 + nrows - 1);
+    const int nr = ke - ks + 1;
+    if (nr <= 0) return;
     const int cbase = kus - col0;
     const int ntaps = kue - kus + 1;
+    if (zero_tol <= 0.0f && nr <= kMaxRows) {
+        // row integral C at row boundaries ks .. ks+nr, in this work-item's LDS column
+        float csum = 0.0f;
+        scan[0] = 0.0f;
+        for (int j = 0; j < nr; ++j) {
+            const float* row = src + (size_t)(ks + j - row0) * stride + cbase;
+            float rk = 0.0f;
+#pragma unroll
+            for (int t = 0; t < kMaxUTaps; ++t) rk = fmaf(wu[t], row[min(t, ntaps - 1)], rk);
+            for (int t = kMaxUTaps; t < ntaps; ++t)
+                rk = fmaf(dd_weight(kus + t, ku0, ku1, ulo, uhi, inv_u, zero_tol), row[t], rk);
+            csum = fmaf(rk, W, csum);
+            scan[(j + 1) * 256] = csum;
+        }
+        const float fks = (float)ks;
+#pragma unroll
+        for (int i = 0; i < ZC; ++i) {
+            float c2[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const float f = (h ? vhi[i] : vlo[i]) - fks;
+                if (f <= 0.0f) {
+                    c2[h] = 0.0f;
+                } else if (f >= (float)nr) {
+                    c2[h] = csum;
+                } else {
+                    const int jj = (int)f;
+                    const float a = scan[jj * 256], b = scan[(jj + 1) * 256];
+                    c2[h] = fmaf(f - (float)jj, b - a, a);
+                }
+            }
+            acc[i] = fmaf(c2[1] - c2[0], inv_v[i], acc[i]);
+        }
+        return;
+    }
     for (int k = ks; k <= ke; ++k) {
         const float* row = src + (size_t)(k - row0) * stride + cbase;
         float rk = 0.0f;
@@ -574,6 +616,8 @@ __global__ void __launch_bounds__(256) dd_backproject_tile_kernel(
     float dx, float dy, float dz, int curved, float scale, float zero_tol, int accumulate,
     int tiles_x, int tiles_y, int tiles_z) {
     __shared__ float win[2][kWinCap];
+    __shared__ float scan_lds[(kMaxRows + 1) * 256];
+    float* scan = scan_lds + threadIdx.x;
     // XCD-aware tile order: workgroup b runs on XCD b % 8; give each XCD a contiguous range
     const int T = tiles_x * tiles_y * tiles_z;
     const int b = blockIdx.x;
@@ -627,21 +671,31 @@ __global__ void __launch_bounds__(256) dd_backproject_tile_kernel(
             tile_window(nxt, xa, xb, ya, yb, za, zb, curved, nu, nv, nv0, nvn, nu0, nun);
             nfits = window_mode(nvn, nun) == 1;
             if (nfits) {
-                const float* qn = q + (size_t)(view + 1) * nv * nu;
+                const float* qn = q + (size_t)(view + 1) * nv * nu + (size_t)nv0 * nu + nu0;
+                // element e = threadIdx.x + 256 p of the window is (row r, column c); step by 256
+                int r = (int)threadIdx.x / nun;
+                int c = (int)threadIdx.x - r * nun;
+                const int dr = 256 / nun, dc = 256 - dr * nun;
 #pragma unroll
                 for (int p = 0; p < kPrefetch; ++p) {
-                    const int e = threadIdx.x + p * 256;
-                    if (e < nvn * nun) pf[p] = qn[(size_t)(nv0 + e / nun) * nu + nu0 + e % nun];
+                    if (r < nvn) pf[p] = qn[(size_t)r * nu + c];
+                    r += dr;
+                    c += dc;
+                    if (c >= nun) {
+                        c -= nun;
+                        ++r;
+                    }
                 }
             }
         }
         if (active && mode != 0) {
             if (mode == 1) {
                 bp_column_view<ZC, true>(cur, win[buf], wun, wv0, wvn, wu0, wun, X, Y, Z0, dz, hx, hy,
-                                         hz, nz_left, curved, scale, zero_tol, nu, nv, acc);
+                                         hz, nz_left, curved, scale, zero_tol, nu, nv, scan, acc);
             } else {
                 bp_column_view<ZC, false>(cur, q + (size_t)view * nv * nu, nu, 0, nv, 0, nu, X, Y, Z0,
-                                          dz, hx, hy, hz, nz_left, curved, scale, zero_tol, nu, nv, acc);
+                                          dz, hx, hy, hz, nz_left, curved, scale, zero_tol, nu, nv, scan,
+                                          acc);
             }
         }
         if (has_next && nfits) {
