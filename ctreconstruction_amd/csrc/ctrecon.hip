@@ -204,10 +204,17 @@ __global__ void __launch_bounds__(256) dd_forward_kernel(
     const float* __restrict__ vol, const float* __restrict__ vol_t, float* __restrict__ proj,
     const float* __restrict__ tabs, const float* __restrict__ rows,
     const int32_t* __restrict__ branch, int nx, int ny, int nz, int nu, int nv, int stride,
-    float zero_tol) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int k = blockIdx.y * blockDim.y + threadIdx.y;
-    const int view = blockIdx.z;
+    float zero_tol, int n_views, int tiles_u, int n_blocks) {
+    // Workgroup b runs on XCD b % 8.  Each XCD gets a contiguous range of (detector tile, view)
+    // pairs, views fastest: concurrent workgroups then cast rays from the same detector tile at
+    // neighbouring angles, which cross the same thin z-slab of the volume (L2/MALL resident).
+    const int b = blockIdx.x;
+    const int xcd = b & 7, local = b >> 3, qd = n_blocks >> 3, rm = n_blocks & 7;
+    const int t = (xcd < rm) ? xcd * (qd + 1) + local : rm * (qd + 1) + (xcd - rm) * qd + local;
+    const int view = t % n_views;
+    const int tile = t / n_views;
+    const int j = (tile % tiles_u) * blockDim.x + threadIdx.x;
+    const int k = (tile / tiles_u) * blockDim.y + threadIdx.y;
     if (j >= nu || k >= nv) return;
     const int br = branch[view];
     const size_t out_idx = ((size_t)view * nv + k) * nu + j;
@@ -234,6 +241,7 @@ __global__ void __launch_bounds__(256) dd_forward_kernel(
     const float Lk = sec * sqrtf(1.0f + s2 * s2);
 
     float acc = 0.0f;
+    const float tol = zero_tol;
     for (int n = 0; n < ns; ++n) {
         const float fn = (float)n;
         const float a1 = fmaf(a_be, fn, a_al);
@@ -248,21 +256,50 @@ __global__ void __launch_bounds__(256) dd_forward_kernel(
         const int kas = max(ka0, 0), kae = min(ka1, nt - 1);
         const int kcs = max(kc0, 0), kce = min(kc1, nz - 1);
         if (kas > kae || kcs > kce) continue;
-        const float inv_a = 1.0f / (ahi - alo);
-        const float inv_c = 1.0f / (chi - clo);
+        // unnormalised overlaps o with the footprint widths: w = o / width (python/GPUFuncs_pycuda.py:427-452)
+        const float wa = ahi - alo, wc = chi - clo;
         const float* slab = src + (size_t)n * s_stride;
         float sum = 0.0f;
-        for (int kc = kcs; kc <= kce; ++kc) {
-            const float wc = dd_weight(kc, kc0, kc1, clo, chi, inv_c, zero_tol);
-            const float* line = slab + (size_t)kc * v_stride;
-            float row = 0.0f;
-            for (int ka = kas; ka <= kae; ++ka) {
-                const float wa = dd_weight(ka, ka0, ka1, alo, ahi, inv_a, zero_tol);
-                row = fmaf(line[ka], wa, row);
+        if (kae - kas <= 2 && kce - kcs <= 2) {
+            // footprint within 3 x 3 cells: fixed taps, clamped addresses, all loads in flight
+            float oa[3], oc[3];
+            int ia[3], ic[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const int ka = kas + i;
+                const float o = fminf(ahi, (float)(ka + 1)) - fmaxf(alo, (float)ka);
+                oa[i] = (ka <= kae && o >= tol * wa) ? o : 0.0f;
+                ia[i] = min(ka, kae);
+                const int kc = kcs + i;
+                const float p = fminf(chi, (float)(kc + 1)) - fmaxf(clo, (float)kc);
+                oc[i] = (kc <= kce && p >= tol * wc) ? p : 0.0f;
+                ic[i] = min(kc, kce) * v_stride;
             }
-            sum = fmaf(row, wc, sum);
+            float f[3][3];
+#pragma unroll
+            for (int jz = 0; jz < 3; ++jz)
+#pragma unroll
+                for (int i = 0; i < 3; ++i) f[jz][i] = slab[ic[jz] + ia[i]];
+#pragma unroll
+            for (int jz = 0; jz < 3; ++jz) {
+                const float row = fmaf(oa[0], f[jz][0], fmaf(oa[1], f[jz][1], oa[2] * f[jz][2]));
+                sum = fmaf(oc[jz], row, sum);
+            }
+        } else {
+            for (int kc = kcs; kc <= kce; ++kc) {
+                const float p = fminf(chi, (float)(kc + 1)) - fmaxf(clo, (float)kc);
+                if (p < tol * wc) continue;
+                const float* line = slab + (size_t)kc * v_stride;
+                float row = 0.0f;
+                for (int ka = kas; ka <= kae; ++ka) {
+                    const float o = fminf(ahi, (float)(ka + 1)) - fmaxf(alo, (float)ka);
+                    if (o >= tol * wa) row = fmaf(line[ka], o, row);
+                }
+                sum = fmaf(row, p, sum);
+            }
         }
-        acc += sum;
+        // a single-cell footprint has o == width, i.e. weight 1 as the reference
+        acc = fmaf(sum, __builtin_amdgcn_rcpf(wa * wc), acc);
     }
     proj[out_idx] = acc * Lk;
 }
@@ -467,7 +504,7 @@ __device__ inline void tile_window(const BpView& v, float xa, float xb, float ya
         const float y = (corner & 2) ? yb : ya;
         const float xr = x * v.c + y * v.s;
         const float yr = y * v.c - x * v.s;
-        const float r = 1.0f / (v.Sy - yr);
+        const float r = __builtin_amdgcn_rcpf(v.Sy - yr);
         const float su = (v.Sx - xr) * r;
         smin = fminf(smin, su);
         smax = fmaxf(smax, su);
@@ -510,7 +547,7 @@ __device__ inline void bp_column_view(const BpView& v, const float* __restrict__
         const float sy = (corner & 2) ? -hy : hy;
         const float xr = xr0 + sx * v.c + sy * v.s;
         const float yr = yr0 + sy * v.c - sx * v.s;
-        const float r = 1.0f / (v.Sy - yr);
+        const float r = __builtin_amdgcn_rcpf(v.Sy - yr);
         const float su = (v.Sx - xr) * r;
         smin = fminf(smin, su);
         smax = fmaxf(smax, su);
@@ -523,7 +560,7 @@ __device__ inline void bp_column_view(const BpView& v, const float* __restrict__
     // taps inside the detector and the source window
     const int kus = max(max(ku0, 0), col0), kue = min(min(ku1, nu - 1), col0 + ncols - 1);
     if (kus > kue) return;
-    const float inv_u = 1.0f / (uhi - ulo);
+    const float inv_u = __builtin_amdgcn_rcpf(uhi - ulo);
     float wu[kMaxUTaps];
 #pragma unroll
     for (int j = 0; j < kMaxUTaps; ++j) {
@@ -531,7 +568,7 @@ __device__ inline void bp_column_view(const BpView& v, const float* __restrict__
         wu[j] = (ku <= kue) ? dd_weight(ku, ku0, ku1, ulo, uhi, inv_u, zero_tol) : 0.0f;
     }
     const float dd = v.R - yr0;
-    const float W = scale * (v.R * v.R) / (dd * dd);
+    const float W = scale * (v.R * v.R) * __builtin_amdgcn_rcpf(dd * dd);
     // v-footprints of the ZC voxels (index space)
     float vlo[ZC], vhi[ZC], inv_v[ZC];
     float rlo = 3.0e38f, rhi = -3.0e38f;
@@ -544,7 +581,7 @@ __device__ inline void bp_column_view(const BpView& v, const float* __restrict__
         const float g1 = v.Av - v.Bv * smn, g2 = v.Av - v.Bv * smx;
         vlo[i] = fminf(g1, g2);
         vhi[i] = fmaxf(g1, g2);
-        inv_v[i] = 1.0f / (vhi[i] - vlo[i]);
+        inv_v[i] = __builtin_amdgcn_rcpf(vhi[i] - vlo[i]);
         if (i < nz_left) {
             rlo = fminf(rlo, vlo[i]);
             rhi = fmaxf(rhi, vhi[i]);
@@ -1000,9 +1037,12 @@ int ctr_dd_forward(const ctr_geometry* g, const double* angles, const int32_t* b
                        dim3(32, 8), 0, s, vol, vol_t, g->nx, g->ny);
     if ((st = status_from_hip(hipGetLastError()))) return st;
     const dim3 block(64, 4);
-    const dim3 grid((g->nu + 63) / 64, (g->nv + 3) / 4, g->n_views);
-    hipLaunchKernelGGL(dd_forward_kernel, grid, block, 0, s, vol, vol_t, proj, tabs, rows, branch,
-                       g->nx, g->ny, g->nz, g->nu, g->nv, stride, 1e-6f);
+    const int tiles_u = (g->nu + 63) / 64, tiles_v = (g->nv + 3) / 4;
+    const long long n_blocks = (long long)tiles_u * tiles_v * g->n_views;
+    if (n_blocks >= 2147483647LL) return CTR_ERR_INVALID;
+    hipLaunchKernelGGL(dd_forward_kernel, dim3((unsigned)n_blocks), block, 0, s, vol, vol_t, proj, tabs,
+                       rows, branch, g->nx, g->ny, g->nz, g->nu, g->nv, stride, 1e-6f, g->n_views,
+                       tiles_u, (int)n_blocks);
     return status_from_hip(hipGetLastError());
 }
 
