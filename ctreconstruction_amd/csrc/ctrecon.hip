@@ -261,25 +261,42 @@ __global__ void __launch_bounds__(256) dd_forward_kernel(
         const float* slab = src + (size_t)n * s_stride;
         float sum = 0.0f;
         if (kae - kas <= 2 && kce - kcs <= 2) {
-            // footprint within 3 x 3 cells: fixed taps, clamped addresses, all loads in flight
+            // footprint within 3 x 3 cells: fixed taps, all loads in flight.  Overlaps use the
+            // footprint clipped to the volume, so taps past its edge get o <= 0; addresses are
+            // clamped into the volume and use 32-bit byte offsets from the uniform slab base.
             float oa[3], oc[3];
-            int ia[3], ic[3];
+            {
+                const float lo = fmaxf(alo, 0.0f), hi = fminf(ahi, (float)nt);
+                const float s1 = floorf(lo) + 1.0f;
+                oa[0] = fminf(hi, s1) - lo;
+                oa[1] = fminf(hi, s1 + 1.0f) - s1;
+                oa[2] = hi - (s1 + 1.0f);
+                const float tl = tol * wa;
 #pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                const int ka = kas + i;
-                const float o = fminf(ahi, (float)(ka + 1)) - fmaxf(alo, (float)ka);
-                oa[i] = (ka <= kae && o >= tol * wa) ? o : 0.0f;
-                ia[i] = min(ka, kae);
-                const int kc = kcs + i;
-                const float p = fminf(chi, (float)(kc + 1)) - fmaxf(clo, (float)kc);
-                oc[i] = (kc <= kce && p >= tol * wc) ? p : 0.0f;
-                ic[i] = min(kc, kce) * v_stride;
+                for (int i = 0; i < 3; ++i) oa[i] = (oa[i] >= tl) ? oa[i] : 0.0f;
             }
+            {
+                const float lo = fmaxf(clo, 0.0f), hi = fminf(chi, (float)nz);
+                const float s1 = floorf(lo) + 1.0f;
+                oc[0] = fminf(hi, s1) - lo;
+                oc[1] = fminf(hi, s1 + 1.0f) - s1;
+                oc[2] = hi - (s1 + 1.0f);
+                const float tl = tol * wc;
+#pragma unroll
+                for (int i = 0; i < 3; ++i) oc[i] = (oc[i] >= tl) ? oc[i] : 0.0f;
+            }
+            const unsigned da = (unsigned)(kae - kas), dc = (unsigned)(kce - kcs);
+            const unsigned vs4 = 4u * (unsigned)v_stride;
+            const unsigned base = 4u * ((unsigned)kcs * (unsigned)v_stride + (unsigned)kas);
+            const unsigned ua[3] = {0u, 4u * min(1u, da), 4u * min(2u, da)};
+            const unsigned uc[3] = {0u, vs4 * min(1u, dc), vs4 * min(2u, dc)};
+            const char* sb = reinterpret_cast<const char*>(slab);
             float f[3][3];
 #pragma unroll
             for (int jz = 0; jz < 3; ++jz)
 #pragma unroll
-                for (int i = 0; i < 3; ++i) f[jz][i] = slab[ic[jz] + ia[i]];
+                for (int i = 0; i < 3; ++i)
+                    f[jz][i] = *reinterpret_cast<const float*>(sb + (base + uc[jz] + ua[i]));
 #pragma unroll
             for (int jz = 0; jz < 3; ++jz) {
                 const float row = fmaf(oa[0], f[jz][0], fmaf(oa[1], f[jz][1], oa[2] * f[jz][2]));
@@ -756,6 +773,145 @@ __global__ void __launch_bounds__(256) dd_backproject_tile_kernel(
 }
 
 // ------------------------------------------------------------------------------------------
+// Matched adjoint of the forward projector
+// ------------------------------------------------------------------------------------------
+
+// The exact transpose of dd_forward_kernel, as a gather over voxels (no atomics).  No reference
+// counterpart: the reference backprojects with the FDK-weighted operator, which is not A^T.
+// For one view a voxel sits in slab n at transverse cell ka and z cell kc (index space of the
+// forward projector's tables).  The detector columns whose U-edge lines bracket [ka, ka+1] at
+// slab n are found by inverting the central projection through the source (edge coordinate
+// of the ray through a point), widened by one column each side; every candidate then gets the
+// forward projector's own overlap weights from the same FP64-built tables, so a tap carries
+// y * L * o_a * o_c / (w_a * w_c) exactly as the forward sum carries it.
+constexpr int kAdjViewStride = 16;
+constexpr int kAdjZChunk = 4;
+
+// Per-view constants (FP64 -> FP32):
+//   [0] branch  [1] Pt: source transverse index coordinate  [2] Ps: source slab coordinate
+//   [3] r = d_t / d_s  [4] eu_t [5] eu_s [6] ew_t [7] ew_s  (detector axes, transverse/slab parts)
+//   [8] SDD  [9] u2_0: coordinate of edge 0  [10] 1/step (mm or rad)
+//   [11] nuA, [12] 1/nuB : row edge e has z index slope nu_e = nuA + nuB * e
+__global__ void adj_geometry_kernel(ctr_geometry g, const double* __restrict__ angles,
+                                    const int32_t* __restrict__ branch, float* __restrict__ at) {
+    const int view = blockIdx.x * blockDim.x + threadIdx.x;
+    if (view >= g.n_views) return;
+    const double th = angles[view];
+    double s, c;
+    sincos(th, &s, &c);
+    const double Sx = -g.sad * s, Sy = g.sad * c;
+    const int br = branch[view];
+    const bool xs = (br == CTR_BRANCH_X);
+    const double ds = xs ? g.dx : -g.dy;
+    const double dt = xs ? -g.dy : g.dx;
+    const int ns = xs ? g.nx : g.ny;
+    const int nt = xs ? g.ny : g.nx;
+    const double rcs = xs ? g.recon_center[0] : g.recon_center[1];
+    const double rct = xs ? g.recon_center[1] : g.recon_center[0];
+    const double Ss = xs ? Sx : Sy, St = xs ? Sy : Sx;
+    const double s0 = rcs - (ns - 1) / 2.0 * ds;
+    // eu = (cos, sin), ew = (sin, -cos) in (x, y)
+    const double eux = c, euy = s, ewx = s, ewy = -c;
+    const double step = (g.detector_shape == CTR_DETECTOR_FLAT) ? g.du : g.du / g.sdd;
+    float* T = at + (size_t)view * kAdjViewStride;
+    T[0] = (float)br;
+    T[1] = (float)((St - rct) / dt + nt / 2.0);
+    T[2] = (float)((Ss - s0) / ds);
+    T[3] = (float)(dt / ds);
+    T[4] = (float)(xs ? euy : eux);
+    T[5] = (float)(xs ? eux : euy);
+    T[6] = (float)(xs ? ewy : ewx);
+    T[7] = (float)(xs ? ewx : ewy);
+    T[8] = (float)g.sdd;
+    T[9] = (float)(-(g.nu - 1) / 2.0 * step - step / 2.0 + g.det_offset[0]);
+    T[10] = (float)(1.0 / step);
+    // row edges: v2_e = -(e - (nv-1)/2) dv + half + off, nu_e = v2_e / (-dz)
+    const double half = (g.detector_shape == CTR_DETECTOR_FLAT) ? 0.5 * g.dv : -0.5 * g.dv;
+    const double nuA = ((g.nv - 1) / 2.0 * g.dv + half + g.det_offset[1]) / (-g.dz);
+    const double nuB = (-g.dv) / (-g.dz);
+    T[11] = (float)nuA;
+    T[12] = (float)(1.0 / nuB);
+}
+
+// Edge coordinate (fractional edge index) of the ray from the source through the point at
+// transverse index coordinate tk and slab coordinate n.
+__device__ inline float adj_edge_coord(const float* A, float tk, float n, int curved) {
+    const float m = (tk - A[1]) * A[3] * __builtin_amdgcn_rcpf(n - A[2]);
+    const float num = A[6] - m * A[7];
+    const float den = m * A[5] - A[4];
+    const float u2 = curved ? atanf(num * __builtin_amdgcn_rcpf(den)) : A[8] * num * __builtin_amdgcn_rcpf(den);
+    return (u2 - A[9]) * A[10];
+}
+
+__global__ void __launch_bounds__(256) dd_adjoint_kernel(
+    const float* __restrict__ y, float* __restrict__ vol, const float* __restrict__ tabs,
+    const float* __restrict__ rows, const float* __restrict__ at, int nx, int ny, int nz, int nu,
+    int nv, int n_views, int stride, int curved, float scale, float zero_tol, int accumulate) {
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+    const int iz0 = blockIdx.z * kAdjZChunk;
+    if (ix >= nx || iy >= ny) return;
+    const int nzc = min(kAdjZChunk, nz - iz0);
+    float acc[kAdjZChunk];
+#pragma unroll
+    for (int i = 0; i < kAdjZChunk; ++i) acc[i] = 0.0f;
+    const float tol = zero_tol;
+    for (int view = 0; view < n_views; ++view) {
+        const float* A = at + (size_t)view * kAdjViewStride;
+        const int br = (int)A[0];
+        if (br != CTR_BRANCH_X && br != CTR_BRANCH_Y) continue;
+        const bool xs = (br == CTR_BRANCH_X);
+        const int n = xs ? ix : iy;     // slab
+        const int ka = xs ? iy : ix;    // transverse cell
+        const float fn = (float)n, fka = (float)ka;
+        const float e1 = adj_edge_coord(A, fka, fn, curved);
+        const float e2 = adj_edge_coord(A, fka + 1.0f, fn, curved);
+        const int j0 = max((int)floorf(fminf(e1, e2)) - 1, 0);
+        const int j1 = min((int)floorf(fmaxf(e1, e2)) + 1, nu - 1);
+        const float* T = tabs + (size_t)view * stride;
+        const float gam = T[6 * nu + 2];
+        const float* yv = y + (size_t)view * nv * nu;
+        for (int j = j0; j <= j1; ++j) {
+            const float a1 = fmaf(T[nu + 1 + j], fn, T[j]);
+            const float a2 = fmaf(T[nu + 2 + j], fn, T[j + 1]);
+            const float alo = fminf(a1, a2), ahi = fmaxf(a1, a2);
+            const float wa = ahi - alo;
+            const float oa = fminf(ahi, fka + 1.0f) - fmaxf(alo, fka);
+            if (!(oa > 0.0f) || oa < tol * wa) continue;
+            const float t = fmaf(T[3 * nu + 2 + j], fn, T[2 * nu + 2 + j]);
+            const float sec = T[4 * nu + 2 + j], inv = T[5 * nu + 2 + j];
+            const float rt = __builtin_amdgcn_rcpf(t);
+            for (int i = 0; i < nzc; ++i) {
+                const float fkc = (float)(iz0 + i);
+                // rows whose edges bracket [kc, kc+1]: nu_e = (c - gam) / t, e = (nu_e - nuA) / nuB
+                const float r1 = ((fkc - gam) * rt - A[11]) * A[12];
+                const float r2 = ((fkc + 1.0f - gam) * rt - A[11]) * A[12];
+                const int k0 = max((int)floorf(fminf(r1, r2)) - 1, 0);
+                const int k1 = min((int)floorf(fmaxf(r1, r2)) + 1, nv - 1);
+                float sum = 0.0f;
+                for (int k = k0; k <= k1; ++k) {
+                    const float c1 = fmaf(rows[k + 1], t, gam);
+                    const float c2 = fmaf(rows[k], t, gam);
+                    const float clo = fminf(c1, c2), chi = fmaxf(c1, c2);
+                    const float wc = chi - clo;
+                    const float oc = fminf(chi, fkc + 1.0f) - fmaxf(clo, fkc);
+                    if (!(oc > 0.0f) || oc < tol * wc) continue;
+                    const float s2 = rows[nv + 1 + k] * inv;
+                    const float L = sec * sqrtf(1.0f + s2 * s2);
+                    sum = fmaf(yv[(size_t)k * nu + j] * L, oc * __builtin_amdgcn_rcpf(wc), sum);
+                }
+                acc[i] = fmaf(sum, oa * __builtin_amdgcn_rcpf(wa), acc[i]);
+            }
+        }
+    }
+    for (int i = 0; i < nzc; ++i) {
+        const size_t idx = ((size_t)(iz0 + i) * ny + iy) * nx + ix;
+        const float v = acc[i] * scale;
+        vol[idx] = accumulate ? vol[idx] + v : v;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
 // FDK filter
 // ------------------------------------------------------------------------------------------
 
@@ -1098,22 +1254,60 @@ int ctr_fdk_filter(const ctr_geometry* g, const double* spectrum, const float* p
         (float)(g->det_offset[1] * mag), (float)(tau / N)));
 }
 
+// Adjoint workspace: the forward projector's tables and row table (no transposed volume),
+// then the per-view adjoint constants.
+struct AdjLayout {
+    size_t tabs_off, rows_off, at_off, total;
+};
+
+AdjLayout adj_layout(const ctr_geometry* g) {
+    AdjLayout L;
+    L.tabs_off = 0;
+    L.rows_off = align_up((size_t)fp_view_stride(g->nu) * g->n_views * 4, 256);
+    L.at_off = align_up(L.rows_off + (size_t)(2 * g->nv + 1) * 4, 256);
+    L.total = align_up(L.at_off + (size_t)g->n_views * kAdjViewStride * 4, 256);
+    return L;
+}
+
 size_t ctr_dd_backproject_workspace_bytes(const ctr_geometry* g, int mode) {
     if (check_geometry(g)) return 0;
-    (void)mode;
+    if (mode == CTR_BP_ADJOINT) return adj_layout(g).total;
     return align_up((size_t)g->n_views * kBpViewStride * sizeof(float), 256);
 }
 
 int ctr_dd_backproject(const ctr_geometry* g, const double* angles, const int32_t* branch,
                        const float* proj, float* vol, int mode, float scale, float zero_tol,
                        int accumulate, void* workspace, size_t workspace_bytes, void* stream) {
-    (void)branch;
     int st = check_geometry(g);
     if (st) return st;
     if (!angles || !proj || !vol || !workspace) return CTR_ERR_INVALID;
-    if (mode != CTR_BP_FDK_REF && mode != CTR_BP_FDK_EDGE) return CTR_ERR_UNSUPPORTED;
+    if (mode != CTR_BP_FDK_REF && mode != CTR_BP_FDK_EDGE && mode != CTR_BP_ADJOINT)
+        return CTR_ERR_INVALID;
     if (workspace_bytes < ctr_dd_backproject_workspace_bytes(g, mode)) return CTR_ERR_WORKSPACE;
     hipStream_t s = (hipStream_t)stream;
+    if (mode == CTR_BP_ADJOINT) {
+        // transpose of ctr_dd_forward: its weights (and its 1e-6 weight zeroing) by construction
+        if (!branch) return CTR_ERR_INVALID;
+        const AdjLayout L = adj_layout(g);
+        char* ws = (char*)workspace;
+        float* tabs = (float*)(ws + L.tabs_off);
+        float* rows = (float*)(ws + L.rows_off);
+        float* at = (float*)(ws + L.at_off);
+        const int stride = fp_view_stride(g->nu);
+        const int gx = (std::max(g->nu + 1, g->nv + 1) + 255) / 256;
+        hipLaunchKernelGGL(fp_geometry_kernel, dim3(gx, g->n_views), dim3(256), 0, s, *g, angles,
+                           branch, tabs, rows, stride);
+        if ((st = status_from_hip(hipGetLastError()))) return st;
+        hipLaunchKernelGGL(adj_geometry_kernel, dim3((g->n_views + 255) / 256), dim3(256), 0, s, *g,
+                           angles, branch, at);
+        if ((st = status_from_hip(hipGetLastError()))) return st;
+        const dim3 block(64, 4);
+        const dim3 grid((g->nx + 63) / 64, (g->ny + 3) / 4, (g->nz + kAdjZChunk - 1) / kAdjZChunk);
+        hipLaunchKernelGGL(dd_adjoint_kernel, grid, block, 0, s, proj, vol, tabs, rows, at, g->nx,
+                           g->ny, g->nz, g->nu, g->nv, g->n_views, stride,
+                           g->detector_shape == CTR_DETECTOR_CURVED ? 1 : 0, scale, 1e-6f, accumulate);
+        return status_from_hip(hipGetLastError());
+    }
     float* vt = (float*)workspace;
     hipLaunchKernelGGL(bp_geometry_kernel, dim3((g->n_views + 255) / 256), dim3(256), 0, s, *g,
                        angles, vt, mode);

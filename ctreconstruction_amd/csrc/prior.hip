@@ -1,0 +1,295 @@
+// prior.hip -- gfx950 kernels of the learned 2.5D residual prior (no reference counterpart).
+//
+// 3x3 "same" convolutions over NHWC bfloat16 activations with bfloat16 weights [Cout][3][3][Cin]
+// and FP32 accumulation:
+//   conv3x3_fwd_kernel    implicit GEMM on MFMA (v_mfma_f32_16x16x32_bf16): M = pixels,
+//                         N = Cout, K = 9*Cin; epilogue adds bias, optional ReLU, optional
+//                         residual, stores bfloat16.  The input gradient of a convolution is
+//                         the same kernel applied to the output gradient with the weights
+//                         flipped in space and transposed in channels.
+//   conv3x3_wgrad_kernel  weight and bias gradients: per pixel chunk, 64-pixel tiles of the
+//                         output gradient and of the shifted input staged in LDS, FP32 outer
+//                         products per (tap, 4 x 4 channel block); partial sums per chunk
+//   conv3x3_wgrad_reduce  sums the chunk partials in a fixed order (deterministic)
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "ctrecon.h"
+
+namespace {
+
+typedef __bf16 bf16_t;
+typedef bf16_t bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kConvWaves = 4;    // waves per workgroup
+constexpr int kPxPerWave = 32;   // two 16-pixel MFMA row tiles per wave
+constexpr int kMaxCout = 64;     // four 16-channel MFMA column tiles
+
+__device__ inline bf16x8 zero8() {
+    bf16x8 z;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) z[e] = (bf16_t)0.0f;
+    return z;
+}
+
+// MFMA operand layout (16 x 16 x 32, wave64): lane l supplies row (l % 16) of A and column
+// (l % 16) of B, both with the 8 consecutive K elements 8*(l / 16) .. 8*(l / 16) + 7; it receives
+// rows 4*(l / 16) .. +3 of column (l % 16) of D.
+template <bool FAST>
+__global__ void __launch_bounds__(256) conv3x3_fwd_kernel(
+    const bf16_t* __restrict__ in, const bf16_t* __restrict__ w, const float* __restrict__ bias,
+    const bf16_t* __restrict__ resid, bf16_t* __restrict__ out, int N, int H, int W, int Cin,
+    int Cout, int relu) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int P = N * H * W;
+    const int pbase = (blockIdx.x * kConvWaves + wave) * kPxPerWave;
+    const int m = lane & 15, g = lane >> 4;
+    const int K = 9 * Cin;
+    const int ksteps = (K + 31) / 32;
+    int py[2], px[2], pn[2];
+    bool pv[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int p = pbase + h * 16 + m;
+        pv[h] = p < P;
+        const int pp = pv[h] ? p : 0;
+        px[h] = pp % W;
+        const int t = pp / W;
+        py[h] = t % H;
+        pn[h] = t / H;
+    }
+    f32x4 acc[2][4];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[h][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+
+    for (int s = 0; s < ksteps; ++s) {
+        const int k0 = s * 32 + g * 8;
+        bf16x8 a[2], b[4];
+        if constexpr (FAST) {
+            // Cin % 32 == 0: the 8 K elements are 8 channels of one tap, 16-byte aligned
+            const int tap = k0 / Cin, ci = k0 - tap * Cin;
+            const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int yy = py[h] + dy, xx = px[h] + dx;
+                const bool ok = pv[h] && yy >= 0 && yy < H && xx >= 0 && xx < W;
+                a[h] = ok ? *reinterpret_cast<const bf16x8*>(
+                                in + ((size_t)(pn[h] * H + yy) * W + xx) * Cin + ci)
+                          : zero8();
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int co = t * 16 + m;
+                b[t] = (co < Cout) ? *reinterpret_cast<const bf16x8*>(w + (size_t)co * K + k0) : zero8();
+            }
+        } else {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) a[h] = zero8();
+#pragma unroll
+            for (int t = 0; t < 4; ++t) b[t] = zero8();
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int k = k0 + e;
+                if (k >= K) break;
+                const int tap = k / Cin, ci = k - tap * Cin;
+                const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int yy = py[h] + dy, xx = px[h] + dx;
+                    if (pv[h] && yy >= 0 && yy < H && xx >= 0 && xx < W)
+                        a[h][e] = in[((size_t)(pn[h] * H + yy) * W + xx) * Cin + ci];
+                }
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int co = t * 16 + m;
+                    if (co < Cout) b[t][e] = w[(size_t)co * K + k];
+                }
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                acc[h][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[h], b[t], acc[h][t], 0, 0, 0);
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int co = t * 16 + m;
+        if (co >= Cout) continue;
+        const float bc = bias ? bias[co] : 0.0f;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int p = pbase + h * 16 + 4 * g + r;
+                if (p >= P) continue;
+                float v = acc[h][t][r] + bc;
+                if (relu) v = fmaxf(v, 0.0f);
+                const size_t o = (size_t)p * Cout + co;
+                if (resid) v += (float)resid[o];
+                out[o] = (bf16_t)v;
+            }
+        }
+    }
+}
+
+constexpr int kWgTile = 64;   // pixels per LDS tile
+constexpr int kWgPad = 65;
+
+__global__ void __launch_bounds__(256) conv3x3_wgrad_kernel(
+    const bf16_t* __restrict__ in, const bf16_t* __restrict__ gout, float* __restrict__ part_w,
+    float* __restrict__ part_b, int N, int H, int W, int Cin, int Cout, int chunk_px) {
+    __shared__ float sy[kWgTile][kWgPad];
+    __shared__ float sx[kWgTile][kWgPad];
+    const int tap = blockIdx.y;
+    const int chunk = blockIdx.x;
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    const int P = N * H * W;
+    const int tid = threadIdx.x;
+    const int co4 = (tid >> 4) * 4, ci4 = (tid & 15) * 4;
+    float acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0f;
+    float accb = 0.0f;
+    const int p_begin = chunk * chunk_px;
+    const int p_end = min(p_begin + chunk_px, P);
+    for (int p0 = p_begin; p0 < p_end; p0 += kWgTile) {
+        // stage: 64 pixels x 64 channels of the output gradient and of the shifted input
+        for (int e = tid; e < kWgTile * kMaxCout; e += 256) {
+            const int pl = e >> 6, c = e & 63;
+            const int p = p0 + pl;
+            float vy = 0.0f, vx = 0.0f;
+            if (p < p_end) {
+                if (c < Cout) vy = (float)gout[(size_t)p * Cout + c];
+                if (c < Cin) {
+                    const int x = p % W;
+                    const int t = p / W;
+                    const int y = t % H, n = t / H;
+                    const int yy = y + dy, xx = x + dx;
+                    if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+                        vx = (float)in[((size_t)(n * H + yy) * W + xx) * Cin + c];
+                }
+            }
+            sy[pl][c] = vy;
+            sx[pl][c] = vx;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int pl = 0; pl < kWgTile; ++pl) {
+            float yv[4], xv[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                yv[i] = sy[pl][co4 + i];
+                xv[i] = sx[pl][ci4 + i];
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(yv[i], xv[j], acc[i][j]);
+            if (tap == 0 && tid < kMaxCout) accb += sy[pl][tid];
+        }
+        __syncthreads();
+    }
+    float* pw = part_w + ((size_t)chunk * 9 + tap) * kMaxCout * kMaxCout;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) pw[(co4 + i) * kMaxCout + ci4 + j] = acc[i][j];
+    if (tap == 0 && tid < kMaxCout) part_b[(size_t)chunk * kMaxCout + tid] = accb;
+}
+
+// grad_w[co][tap][ci] = sum over chunks (fixed order); grad_b[co] likewise
+__global__ void conv3x3_wgrad_reduce(const float* __restrict__ part_w, const float* __restrict__ part_b,
+                                     float* __restrict__ grad_w, float* __restrict__ grad_b,
+                                     int n_chunks, int Cin, int Cout) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int total = Cout * 9 * Cin;
+    if (idx < total) {
+        const int ci = idx % Cin;
+        const int tap = (idx / Cin) % 9;
+        const int co = idx / (9 * Cin);
+        float s = 0.0f;
+        for (int c = 0; c < n_chunks; ++c)
+            s += part_w[(((size_t)c * 9 + tap) * kMaxCout + co) * kMaxCout + ci];
+        grad_w[idx] = s;
+    }
+    if (grad_b && idx < Cout) {
+        float s = 0.0f;
+        for (int c = 0; c < n_chunks; ++c) s += part_b[(size_t)c * kMaxCout + idx];
+        grad_b[idx] = s;
+    }
+}
+
+int conv_chunks(long long P) {
+    long long c = (P + 2047) / 2048;
+    if (c > 512) c = 512;
+    if (c < 1) c = 1;
+    return (int)c;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ctr_conv3x3_bf16_forward(const void* in, const void* weight, const float* bias,
+                             const void* residual, void* out, int n, int h, int w, int cin, int cout,
+                             int relu, void* stream) {
+    if (!in || !weight || !out) return CTR_ERR_INVALID;
+    if (n <= 0 || h <= 0 || w <= 0 || cin <= 0 || cout <= 0 || cout > kMaxCout) return CTR_ERR_INVALID;
+    const long long P = (long long)n * h * w;
+    if (P * (cin > cout ? cin : cout) >= 2147483647LL) return CTR_ERR_INVALID;
+    const long long blocks = (P + kConvWaves * kPxPerWave - 1) / (kConvWaves * kPxPerWave);
+    hipStream_t s = (hipStream_t)stream;
+    const bf16_t* x = (const bf16_t*)in;
+    const bf16_t* wt = (const bf16_t*)weight;
+    const bf16_t* r = (const bf16_t*)residual;
+    bf16_t* y = (bf16_t*)out;
+    if (cin % 32 == 0) {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(conv3x3_fwd_kernel<true>), dim3((unsigned)blocks), dim3(256),
+                           0, s, x, wt, bias, r, y, n, h, w, cin, cout, relu);
+    } else {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(conv3x3_fwd_kernel<false>), dim3((unsigned)blocks), dim3(256),
+                           0, s, x, wt, bias, r, y, n, h, w, cin, cout, relu);
+    }
+    return hipGetLastError() == hipSuccess ? CTR_OK : CTR_ERR_HIP;
+}
+
+size_t ctr_conv3x3_bf16_wgrad_workspace_bytes(int n, int h, int w, int cin, int cout) {
+    (void)cin;
+    (void)cout;
+    if (n <= 0 || h <= 0 || w <= 0) return 0;
+    const int chunks = conv_chunks((long long)n * h * w);
+    return (size_t)chunks * (9 * kMaxCout * kMaxCout + kMaxCout) * sizeof(float);
+}
+
+int ctr_conv3x3_bf16_wgrad(const void* in, const void* grad_out, float* grad_weight, float* grad_bias,
+                           int n, int h, int w, int cin, int cout, void* workspace,
+                           size_t workspace_bytes, void* stream) {
+    if (!in || !grad_out || !grad_weight || !workspace) return CTR_ERR_INVALID;
+    if (n <= 0 || h <= 0 || w <= 0 || cin <= 0 || cout <= 0 || cin > kMaxCout || cout > kMaxCout)
+        return CTR_ERR_INVALID;
+    const long long P = (long long)n * h * w;
+    if (P * kMaxCout >= 2147483647LL) return CTR_ERR_INVALID;
+    if (workspace_bytes < ctr_conv3x3_bf16_wgrad_workspace_bytes(n, h, w, cin, cout)) return CTR_ERR_WORKSPACE;
+    const int chunks = conv_chunks(P);
+    const int chunk_px = (int)(((P + chunks - 1) / chunks + kWgTile - 1) / kWgTile * kWgTile);
+    float* part_w = (float*)workspace;
+    float* part_b = part_w + (size_t)chunks * 9 * kMaxCout * kMaxCout;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(conv3x3_wgrad_kernel, dim3(chunks, 9), dim3(256), 0, s, (const bf16_t*)in,
+                       (const bf16_t*)grad_out, part_w, part_b, n, h, w, cin, cout, chunk_px);
+    if (hipGetLastError() != hipSuccess) return CTR_ERR_HIP;
+    const int total = cout * 9 * cin;
+    hipLaunchKernelGGL(conv3x3_wgrad_reduce, dim3((total + 255) / 256), dim3(256), 0, s, part_w, part_b,
+                       grad_weight, grad_bias, chunks, cin, cout);
+    return hipGetLastError() == hipSuccess ? CTR_OK : CTR_ERR_HIP;
+}
+
+}  // extern "C"

@@ -109,12 +109,16 @@ def dd_backproject(proj: torch.Tensor, geom: ScanGeometry, mode: str = "ref", sc
 
     ``mode='ref'``: the reference's FDK backprojector with its half-pixel detector grid
     (python/Reconstruction_pycuda.py:320-435, kernel python/GPUFuncs_pycuda.py:837-956);
-    ``mode='edge'``: the same with the forward projector's cell edges.  ``scale`` multiplies
-    the data (the reference uses Q = proj * dtheta, which is the default here).
+    ``mode='edge'``: the same with the forward projector's cell edges; ``mode='adjoint'``: the
+    exact transpose A^T of :func:`dd_forward` (no reference counterpart).  ``scale`` multiplies
+    the result; it defaults to dtheta for the FDK modes (the reference's Q = proj * dtheta) and
+    to 1 for the adjoint.
     """
-    if mode not in ("ref", "edge"):
-        raise ValueError(f"mode must be 'ref' or 'edge', got {mode!r}")
+    if mode not in N.BP_MODES:
+        raise ValueError(f"mode must be one of {sorted(N.BP_MODES)}, got {mode!r}")
     _require(proj, geom.proj_shape, "proj")
+    if mode == "adjoint":
+        geom.check_supported()
     dev = proj.device
     if out is None:
         out = torch.empty(geom.vol_shape, dtype=torch.float32, device=dev)
@@ -126,11 +130,17 @@ def dd_backproject(proj: torch.Tensor, geom: ScanGeometry, mode: str = "ref", sc
     m = N.BP_MODES[mode]
     nbytes = lib.ctr_dd_backproject_workspace_bytes(ctypes.byref(g), m)
     ws = _workspace(nbytes, dev)
-    sc = geom.dtheta if scale is None else float(scale)
+    if scale is None:
+        scale = 1.0 if mode == "adjoint" else geom.dtheta
     N.check(lib.ctr_dd_backproject(ctypes.byref(g), _ptr(ang), _ptr(br), _ptr(proj), _ptr(out), m,
-                                   sc, float(zero_tol), int(bool(accumulate)), _ptr(ws), nbytes,
-                                   _stream(dev)), "ctr_dd_backproject")
+                                   float(scale), float(zero_tol), int(bool(accumulate)), _ptr(ws),
+                                   nbytes, _stream(dev)), "ctr_dd_backproject")
     return out
+
+
+def dd_adjoint(proj: torch.Tensor, geom: ScanGeometry, out: torch.Tensor | None = None) -> torch.Tensor:
+    """A^T proj: exact transpose of :func:`dd_forward` (matched backprojector)."""
+    return dd_backproject(proj, geom, mode="adjoint", out=out)
 
 
 def fdk(proj: torch.Tensor, geom: ScanGeometry, mode: str = "ref",

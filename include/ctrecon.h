@@ -122,11 +122,31 @@ int ctr_fdk_filter(const ctr_geometry* g, const double* spectrum, const float* p
  * Voxel-driven gather over all views in one launch, no atomics.
  * vol = (accumulate ? vol : 0) + sum_views scale * BP_view(proj); the reference's FDK uses
  * scale = dtheta (Q = proj * dtheta, :411).  Overlap weights below `zero_tol` are zeroed
- * (0 = never, as the pycuda kernel; 1e-5 = the numba kernels). */
+ * (0 = never, as the pycuda kernel; 1e-5 = the numba kernels).
+ * mode CTR_BP_ADJOINT computes scale * A^T proj, the exact transpose of ctr_dd_forward (its
+ * weights and its 1e-6 zeroing; `zero_tol` is ignored); it needs `branch` and has no
+ * counterpart in the reference, which only has the FDK-weighted backprojector. */
 size_t ctr_dd_backproject_workspace_bytes(const ctr_geometry* g, int mode);
 int ctr_dd_backproject(const ctr_geometry* g, const double* angles, const int32_t* branch,
                        const float* proj, float* vol, int mode, float scale, float zero_tol,
                        int accumulate, void* workspace, size_t workspace_bytes, void* stream);
+
+/* ---- Learned prior: 3x3 convolutions, bfloat16 on MFMA, FP32 accumulation ---------------
+ * No counterpart in the reference (the learned residual prior is new in this build).
+ * Activations are NHWC bfloat16 [n][h][w][c]; weights bfloat16 [cout][3][3][cin]; bias float32.
+ * ctr_conv3x3_bf16_forward: out = act(conv(in, weight) + bias) + residual ("same" zero
+ * padding; act = ReLU if relu != 0; bias and residual may be NULL); cout <= 64.
+ * The input gradient is ctr_conv3x3_bf16_forward applied to the output gradient with the
+ * weights flipped in space and transposed in channels.
+ * ctr_conv3x3_bf16_wgrad: grad_weight[cout][3][3][cin] (float32) and grad_bias[cout] (may be
+ * NULL) summed over all pixels, in a fixed order; cin, cout <= 64. */
+int ctr_conv3x3_bf16_forward(const void* in, const void* weight, const float* bias,
+                             const void* residual, void* out, int n, int h, int w, int cin,
+                             int cout, int relu, void* stream);
+size_t ctr_conv3x3_bf16_wgrad_workspace_bytes(int n, int h, int w, int cin, int cout);
+int ctr_conv3x3_bf16_wgrad(const void* in, const void* grad_out, float* grad_weight,
+                           float* grad_bias, int n, int h, int w, int cin, int cout,
+                           void* workspace, size_t workspace_bytes, void* stream);
 
 #ifdef __cplusplus
 }
