@@ -26,10 +26,10 @@ from .autograd import DDProjectorFunction, adjoint_batch, forward_batch
 from .geometry import ScanGeometry
 
 
-def _bp_ref(proj: torch.Tensor, geom: ScanGeometry) -> torch.Tensor:
+def _bp_ref(proj: torch.Tensor, geom: ScanGeometry, mode: str = "ref") -> torch.Tensor:
     """Reference ``backward()``: FDK-weighted DD backprojection times dtheta (no filter)."""
     def one(p, g, out=None):
-        return P.dd_backproject(p, g, mode="ref", out=out)
+        return P.dd_backproject(p, g, mode=mode, out=out)
     if proj.dim() == 3:
         return one(proj.contiguous(), geom)
     out = torch.empty((proj.shape[0],) + geom.vol_shape, dtype=torch.float32, device=proj.device)
@@ -38,12 +38,25 @@ def _bp_ref(proj: torch.Tensor, geom: ScanGeometry) -> torch.Tensor:
     return out
 
 
+_NORM_CACHE: dict = {}
+
+
 def normalisers(geom: ScanGeometry, backprojector: str, device, eps: float):
-    """(row sums A 1, column sums B 1) for the chosen backprojector B."""
+    """(row sums A 1, column sums B 1) for the chosen backprojector B; cached per geometry."""
+    key = (id(geom), backprojector, str(device))
+    hit = _NORM_CACHE.get(key)
+    if hit is not None and hit[0] is geom:
+        return hit[1], hit[2]
     ones_v = torch.ones(geom.vol_shape, device=device)
     ones_p = torch.ones(geom.proj_shape, device=device)
     a1 = P.dd_forward(ones_v, geom)
-    b1 = P.dd_adjoint(ones_p, geom) if backprojector == "matched" else _bp_ref(ones_p, geom)
+    if backprojector == "matched":
+        b1 = P.dd_adjoint(ones_p, geom)
+    elif backprojector == "edge":
+        b1 = P.dd_backproject(ones_p, geom, mode="edge")
+    else:
+        b1 = _bp_ref(ones_p, geom)
+    _NORM_CACHE[key] = (geom, a1, b1)
     return a1, b1
 
 
@@ -85,7 +98,7 @@ def sart(proj0: torch.Tensor, geom: ScanGeometry, n_iter: int, relaxation: float
             (grad,) = torch.autograd.grad(loss, xv)
             step = -grad
         else:
-            step = _bp_ref((proj0 - forward_batch(x, geom)) * w, geom)
+            step = _bp_ref((proj0 - forward_batch(x, geom)) * w, geom, "edge" if backprojector == "edge" else "ref")
         x = x.detach() + relaxation * step / (b1 + eps)
         if nonneg:
             x = torch.clamp_min(x, 0.0)
