@@ -102,6 +102,23 @@ def cpu_baseline(params, q_gpu, geom, dev, sample_views=(0, 1), slab=8):
     }
 
 
+WEIGHTS = os.path.join(ROOT, "ctreconstruction_amd", "weights", "prior_sparse60.safetensors")
+
+
+def learned_vs_sart(seed=10000):
+    """Sparse-view benchmark (256^3, 60 views, 384^2, Poisson 1e5): wall time for plain SART to
+    reach its 200-iteration RMSE vs the learned prior + SART pipeline (trained weights in
+    ctreconstruction_amd/weights/); None when no trained weights are committed."""
+    if not os.path.exists(WEIGHTS):
+        return None
+    from ctreconstruction_amd import learned as L
+    net = L.load_weights(WEIGHTS)
+    factory = L.PairFactory()
+    res = L.time_to_rmse(factory, net, seed)
+    res["phantom_seed"] = seed
+    return res
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +126,7 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-prior time-to-RMSE leg")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (rocprof counter calibration)")
     args = ap.parse_args(argv)
@@ -206,7 +224,7 @@ def main(argv=None):
         "fdk_backprojection": {"s": bp_s, "gups": updates / bp_s / 1e9},
         "fdk_filter": {"s": sec - bp_s},
         "dd_forward": {"s": fp_s, "gups": updates / fp_s / 1e9, "ray_slab_updates_per_s": ray_slab / fp_s},
-        "sart_vs_learned_time_to_rmse": None,
+        "sart_vs_learned_time_to_rmse": None if args.no_learned else learned_vs_sart(),
         "setup_s": t_setup,
     }
     if not args.no_cpu_baseline:

@@ -44,8 +44,8 @@ int check_geometry(const ctr_geometry* g) {
     const double vox = (double)g->nx * g->ny * g->nz;
     const double cells = (double)g->n_views * g->nv * g->nu;
     if (vox >= 2147483647.0 || cells >= 2147483647.0) return CTR_ERR_INVALID;
-    // grid y/z limits: forward (nv/4, n_views), transpose (ny/32, nz), backprojection (ny/4, nz/8)
-    if (g->n_views > 65535 || g->nv > 65535 * 4 || g->ny > 65535 * 4 || g->nz > 65535)
+    // grid y/z limits: transposes (nz/32, ny), backprojection/adjoint (ny/4, nz/4)
+    if (g->n_views > 65535 || g->nv > 65535 * 4 || g->ny > 65535 || g->nz > 65535)
         return CTR_ERR_INVALID;
     return CTR_OK;
 }
@@ -71,12 +71,15 @@ struct FpLayout {
     size_t tabs_off, rows_off, volt_off, total;
 };
 
+// workspace: tables, row table, then the volume twice with z fastest: [y][x][z] (gathered by
+// y-slab views) and [x][y][z] (x-slab views)
 FpLayout fp_layout(const ctr_geometry* g) {
     FpLayout L;
+    const size_t vbytes = align_up((size_t)g->nx * g->ny * g->nz * 4, 256);
     L.tabs_off = 0;
     L.rows_off = align_up(L.tabs_off + (size_t)fp_view_stride(g->nu) * g->n_views * 4, 256);
     L.volt_off = align_up(L.rows_off + (size_t)(2 * g->nv + 1) * 4, 256);
-    L.total = align_up(L.volt_off + (size_t)g->nx * g->ny * g->nz * 4, 256);
+    L.total = L.volt_off + 2 * vbytes;
     return L;
 }
 
@@ -321,6 +324,153 @@ __global__ void __launch_bounds__(256) dd_forward_kernel(
     proj[out_idx] = acc * Lk;
 }
 
+// vol[z][y][x] -> v_ys[y][x][z] and v_xs[x][y][z]: 32 (x) x 32 (z) tiles of one y-plane
+// through LDS, coalesced along x on read and along z on write.
+__global__ void __launch_bounds__(256) zfast_transpose_kernel(const float* __restrict__ in,
+                                                              float* __restrict__ v_ys,
+                                                              float* __restrict__ v_xs, int nx,
+                                                              int ny, int nz) {
+    __shared__ float tile[32][33];
+    const int x0 = blockIdx.x * 32, z0 = blockIdx.y * 32, y = blockIdx.z;
+    for (int r = threadIdx.y; r < 32; r += 8) {
+        const int z = z0 + r, x = x0 + threadIdx.x;
+        if (z < nz && x < nx) tile[r][threadIdx.x] = in[((size_t)z * ny + y) * nx + x];
+    }
+    __syncthreads();
+    for (int r = threadIdx.y; r < 32; r += 8) {
+        const int x = x0 + r, z = z0 + threadIdx.x;
+        if (x < nx && z < nz) {
+            const float v = tile[threadIdx.x][r];
+            v_ys[((size_t)y * nx + x) * nz + z] = v;
+            v_xs[((size_t)x * ny + y) * nz + z] = v;
+        }
+    }
+}
+
+// DD forward projection with one wavefront per (view, detector column, 64 rows).  All rays of a
+// column share their transverse footprint at every slab (the U-edge lines depend on the column
+// only), so the transverse overlaps, the slab test and the transverse addresses are uniform
+// across the wavefront and computed once for its 64 rays; each lane adds its row's z footprint.
+// The volume is read from a z-fastest copy, so the lanes' taps are consecutive in memory.
+// Same semantics and weights as dd_forward_kernel (python/GPUFuncs_pycuda.py:398-524).
+constexpr int kColWaves = 4;
+
+__global__ void __launch_bounds__(256) dd_forward_column_kernel(
+    const float* __restrict__ v_ys, const float* __restrict__ v_xs, float* __restrict__ proj,
+    const float* __restrict__ tabs, const float* __restrict__ rows,
+    const int32_t* __restrict__ branch, int nx, int ny, int nz, int nu, int nv, int stride,
+    float zero_tol, int n_views, int col_groups, int n_blocks) {
+    const int b = blockIdx.x;
+    const int xcd = b & 7, local = b >> 3, qd = n_blocks >> 3, rm = n_blocks & 7;
+    const int t = (xcd < rm) ? xcd * (qd + 1) + local : rm * (qd + 1) + (xcd - rm) * qd + local;
+    const int view = t % n_views;
+    const int rest = t / n_views;
+    const int cg = rest % col_groups, rb = rest / col_groups;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int j = cg * kColWaves + wave;
+    if (j >= nu) return;
+    const int k = rb * 64 + lane;
+    const bool active = k < nv;
+    const int kk = active ? k : nv - 1;
+    const size_t out_idx = ((size_t)view * nv + kk) * nu + j;
+    const int br = branch[view];
+    if (br != CTR_BRANCH_X && br != CTR_BRANCH_Y) {
+        if (active) proj[out_idx] = 0.0f;
+        return;
+    }
+    const bool xs = (br == CTR_BRANCH_X);
+    const float* __restrict__ src = xs ? v_xs : v_ys;
+    const int ns = xs ? nx : ny;
+    const int nt = xs ? ny : nx;
+    const size_t slab_stride = (size_t)nt * nz;
+    const float* T = tabs + (size_t)view * stride;
+    const float a_al = T[j], b_al = T[j + 1];
+    const float a_be = T[nu + 1 + j], b_be = T[nu + 2 + j];
+    const float T0 = T[2 * nu + 2 + j], dT = T[3 * nu + 2 + j];
+    const float sec = T[4 * nu + 2 + j], inv = T[5 * nu + 2 + j];
+    const float gam = T[6 * nu + 2];
+    const float nu1 = rows[kk + 1], nu2 = rows[kk];
+    const float vc = rows[nv + 1 + kk];
+    const float s2 = vc * inv;
+    const float Lk = sec * sqrtf(1.0f + s2 * s2);
+    const float tol = zero_tol;
+    float acc = 0.0f;
+    for (int n = 0; n < ns; ++n) {
+        const float fn = (float)n;
+        // transverse footprint: uniform across the wavefront
+        const float a1 = fmaf(a_be, fn, a_al);
+        const float a2 = fmaf(b_be, fn, b_al);
+        const float alo = fminf(a1, a2), ahi = fmaxf(a1, a2);
+        const int kas = max((int)floorf(alo), 0), kae = min((int)floorf(ahi), nt - 1);
+        if (kas > kae) continue;
+        const float wa = ahi - alo;
+        // z footprint of this lane's row
+        const float tt = fmaf(dT, fn, T0);
+        const float c1 = fmaf(nu1, tt, gam);
+        const float c2 = fmaf(nu2, tt, gam);
+        const float clo = fminf(c1, c2), chi = fmaxf(c1, c2);
+        const int kcs = max((int)floorf(clo), 0), kce = min((int)floorf(chi), nz - 1);
+        if (kcs > kce) continue;
+        const float wc = chi - clo;
+        const float* slab = src + (size_t)n * slab_stride;
+        float sum = 0.0f;
+        if (kae - kas <= 2 && kce - kcs <= 2) {
+            float oa[3], oc[3];
+            {
+                const float lo = fmaxf(alo, 0.0f), hi = fminf(ahi, (float)nt);
+                const float s1 = floorf(lo) + 1.0f;
+                oa[0] = fminf(hi, s1) - lo;
+                oa[1] = fminf(hi, s1 + 1.0f) - s1;
+                oa[2] = hi - (s1 + 1.0f);
+                const float tl = tol * wa;
+#pragma unroll
+                for (int i = 0; i < 3; ++i) oa[i] = (oa[i] >= tl) ? oa[i] : 0.0f;
+            }
+            {
+                const float lo = fmaxf(clo, 0.0f), hi = fminf(chi, (float)nz);
+                const float s1 = floorf(lo) + 1.0f;
+                oc[0] = fminf(hi, s1) - lo;
+                oc[1] = fminf(hi, s1 + 1.0f) - s1;
+                oc[2] = hi - (s1 + 1.0f);
+                const float tl = tol * wc;
+#pragma unroll
+                for (int i = 0; i < 3; ++i) oc[i] = (oc[i] >= tl) ? oc[i] : 0.0f;
+            }
+            const unsigned da = (unsigned)(kae - kas), dc = (unsigned)(kce - kcs);
+            const unsigned nz4 = 4u * (unsigned)nz;
+            const unsigned base = 4u * ((unsigned)kas * (unsigned)nz + (unsigned)kcs);
+            const unsigned ua[3] = {0u, nz4 * min(1u, da), nz4 * min(2u, da)};
+            const unsigned uc[3] = {0u, 4u * min(1u, dc), 4u * min(2u, dc)};
+            const char* sb = reinterpret_cast<const char*>(slab);
+            float f[3][3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int jz = 0; jz < 3; ++jz)
+                    f[i][jz] = *reinterpret_cast<const float*>(sb + (base + ua[i] + uc[jz]));
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const float col = fmaf(oc[0], f[i][0], fmaf(oc[1], f[i][1], oc[2] * f[i][2]));
+                sum = fmaf(oa[i], col, sum);
+            }
+        } else {
+            for (int ka = kas; ka <= kae; ++ka) {
+                const float o = fminf(ahi, (float)(ka + 1)) - fmaxf(alo, (float)ka);
+                if (o < tol * wa) continue;
+                const float* line = slab + (size_t)ka * nz;
+                float col = 0.0f;
+                for (int kc = kcs; kc <= kce; ++kc) {
+                    const float p = fminf(chi, (float)(kc + 1)) - fmaxf(clo, (float)kc);
+                    if (p >= tol * wc) col = fmaf(line[kc], p, col);
+                }
+                sum = fmaf(col, o, sum);
+            }
+        }
+        acc = fmaf(sum, __builtin_amdgcn_rcpf(wa * wc), acc);
+    }
+    if (active) proj[out_idx] = acc * Lk;
+}
+
 // ------------------------------------------------------------------------------------------
 // Backprojection
 // ------------------------------------------------------------------------------------------
@@ -487,7 +637,7 @@ __global__ void __launch_bounds__(256) dd_backproject_kernel(
 // that XCD's L2.  A window larger than the LDS buffer is gathered from global memory.
 constexpr int kTileX = 16, kTileY = 16;
 constexpr int kWinCap = 1024;          // floats per LDS window buffer
-constexpr int kMaxUTaps = 4;           // u taps held in registers (more are looped)
+constexpr int kMaxUTaps = 3;           // u taps held in registers (more are looped)
 constexpr int kPrefetch = kWinCap / 256;
 constexpr int kMaxRows = 16;           // rows per column chunk on the row-integral path
 
@@ -624,22 +774,18 @@ __device__ inline void bp_column_view(const BpView& v, const float* __restrict__
             csum = fmaf(rk, W, csum);
             scan[(j + 1) * 256] = csum;
         }
-        const float fks = (float)ks;
+        // C(v) between row boundaries is linear; clamping v to [ks, ks + nr] gives C = 0 below
+        // and the total above (rows outside the detector or the window contribute nothing)
+        const float fks = (float)ks, fnr = (float)nr;
 #pragma unroll
         for (int i = 0; i < ZC; ++i) {
             float c2[2];
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
-                const float f = (h ? vhi[i] : vlo[i]) - fks;
-                if (f <= 0.0f) {
-                    c2[h] = 0.0f;
-                } else if (f >= (float)nr) {
-                    c2[h] = csum;
-                } else {
-                    const int jj = (int)f;
-                    const float a = scan[jj * 256], b = scan[(jj + 1) * 256];
-                    c2[h] = fmaf(f - (float)jj, b - a, a);
-                }
+                const float f = fminf(fmaxf((h ? vhi[i] : vlo[i]) - fks, 0.0f), fnr);
+                const int jj = min((int)f, nr - 1);
+                const float a = scan[jj * 256], b = scan[(jj + 1) * 256];
+                c2[h] = fmaf(f - (float)jj, b - a, a);
             }
             acc[i] = fmaf(c2[1] - c2[0], inv_v[i], acc[i]);
         }
@@ -781,7 +927,7 @@ __global__ void __launch_bounds__(256) dd_backproject_tile_kernel(
 // For one view a voxel sits in slab n at transverse cell ka and z cell kc (index space of the
 // forward projector's tables).  The detector columns whose U-edge lines bracket [ka, ka+1] at
 // slab n are found by inverting the central projection through the source (edge coordinate
-// of the ray through a point), widened by one column each side; every candidate then gets the
+// of the ray through a point), with a rounding margin; every candidate then gets the
 // forward projector's own overlap weights from the same FP64-built tables, so a tap carries
 // y * L * o_a * o_c / (w_a * w_c) exactly as the forward sum carries it.
 constexpr int kAdjViewStride = 16;
@@ -866,8 +1012,9 @@ __global__ void __launch_bounds__(256) dd_adjoint_kernel(
         const float fn = (float)n, fka = (float)ka;
         const float e1 = adj_edge_coord(A, fka, fn, curved);
         const float e2 = adj_edge_coord(A, fka + 1.0f, fn, curved);
-        const int j0 = max((int)floorf(fminf(e1, e2)) - 1, 0);
-        const int j1 = min((int)floorf(fmaxf(e1, e2)) + 1, nu - 1);
+        // the inverse projection is accurate to ~1e-5 edge; a 0.01 margin covers its rounding
+        const int j0 = max((int)floorf(fminf(e1, e2) - 0.01f), 0);
+        const int j1 = min((int)floorf(fmaxf(e1, e2) + 0.01f), nu - 1);
         const float* T = tabs + (size_t)view * stride;
         const float gam = T[6 * nu + 2];
         const float* yv = y + (size_t)view * nv * nu;
@@ -886,8 +1033,8 @@ __global__ void __launch_bounds__(256) dd_adjoint_kernel(
                 // rows whose edges bracket [kc, kc+1]: nu_e = (c - gam) / t, e = (nu_e - nuA) / nuB
                 const float r1 = ((fkc - gam) * rt - A[11]) * A[12];
                 const float r2 = ((fkc + 1.0f - gam) * rt - A[11]) * A[12];
-                const int k0 = max((int)floorf(fminf(r1, r2)) - 1, 0);
-                const int k1 = min((int)floorf(fmaxf(r1, r2)) + 1, nv - 1);
+                const int k0 = max((int)floorf(fminf(r1, r2) - 0.01f), 0);
+                const int k1 = min((int)floorf(fmaxf(r1, r2) + 0.01f), nv - 1);
                 float sum = 0.0f;
                 for (int k = k0; k <= k1; ++k) {
                     const float c1 = fmaf(rows[k + 1], t, gam);
@@ -1183,22 +1330,23 @@ int ctr_dd_forward(const ctr_geometry* g, const double* angles, const int32_t* b
     char* ws = (char*)workspace;
     float* tabs = (float*)(ws + L.tabs_off);
     float* rows = (float*)(ws + L.rows_off);
-    float* vol_t = (float*)(ws + L.volt_off);
+    float* v_ys = (float*)(ws + L.volt_off);
+    float* v_xs = v_ys + align_up((size_t)g->nx * g->ny * g->nz * 4, 256) / 4;
     const int stride = fp_view_stride(g->nu);
     const int gx = (std::max(g->nu + 1, g->nv + 1) + 255) / 256;
     hipLaunchKernelGGL(fp_geometry_kernel, dim3(gx, g->n_views), dim3(256), 0, s, *g, angles, branch,
                        tabs, rows, stride);
     if ((st = status_from_hip(hipGetLastError()))) return st;
-    hipLaunchKernelGGL(transpose_xy_kernel, dim3((g->nx + 31) / 32, (g->ny + 31) / 32, g->nz),
-                       dim3(32, 8), 0, s, vol, vol_t, g->nx, g->ny);
+    hipLaunchKernelGGL(zfast_transpose_kernel, dim3((g->nx + 31) / 32, (g->nz + 31) / 32, g->ny),
+                       dim3(32, 8), 0, s, vol, v_ys, v_xs, g->nx, g->ny, g->nz);
     if ((st = status_from_hip(hipGetLastError()))) return st;
-    const dim3 block(64, 4);
-    const int tiles_u = (g->nu + 63) / 64, tiles_v = (g->nv + 3) / 4;
-    const long long n_blocks = (long long)tiles_u * tiles_v * g->n_views;
+    const int col_groups = (g->nu + kColWaves - 1) / kColWaves;
+    const int row_blocks = (g->nv + 63) / 64;
+    const long long n_blocks = (long long)col_groups * row_blocks * g->n_views;
     if (n_blocks >= 2147483647LL) return CTR_ERR_INVALID;
-    hipLaunchKernelGGL(dd_forward_kernel, dim3((unsigned)n_blocks), block, 0, s, vol, vol_t, proj, tabs,
-                       rows, branch, g->nx, g->ny, g->nz, g->nu, g->nv, stride, 1e-6f, g->n_views,
-                       tiles_u, (int)n_blocks);
+    hipLaunchKernelGGL(dd_forward_column_kernel, dim3((unsigned)n_blocks), dim3(256), 0, s, v_ys, v_xs,
+                       proj, tabs, rows, branch, g->nx, g->ny, g->nz, g->nu, g->nv, stride, 1e-6f,
+                       g->n_views, col_groups, (int)n_blocks);
     return status_from_hip(hipGetLastError());
 }
 
