@@ -58,6 +58,13 @@ class Reconstruction:
         self.proj = None
         self._spectrum = None
 
+    @classmethod
+    def from_params_file(cls, filename, device=None, **overrides) -> "Reconstruction":
+        """Geometry from a ``key: value`` text file (python/ReconstructionParams.txt:1-11, parsed as
+        python/ConeBeam.py:51-83); voxel centres match python/ConeBeam.py:103-112."""
+        from .paramsfile import load_recon_params
+        return cls(load_recon_params(filename, **overrides), device=device)
+
     # ------------------------------------------------------------------ raw file I/O (:101-111)
     def LoadProj(self, filename, dtype=np.float32):
         self.proj = np.fromfile(filename, dtype=dtype).reshape([self.nView, self.nv, self.nu])
