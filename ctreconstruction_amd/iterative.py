@@ -17,8 +17,6 @@ All functions accept one volume [nz, ny, nx] (projections [nView, nv, nu]) or a 
 """
 from __future__ import annotations
 
-import math
-
 import torch
 
 from . import projector as P
@@ -168,13 +166,18 @@ def asd_pocs(proj0: torch.Tensor, geom: ScanGeometry, n_outer: int = 20, na: int
              backprojector: str = "ref", tv: str = "reference", callback=None) -> torch.Tensor:
     """ASD-POCS (python/ASD_POCS.py:130-202): ``na`` non-negative SART steps, then ``ng``
     normalised TV steepest-descent steps of size alpha * |dx_POCS|, with the reference's
-    adaptive rules for alpha and beta.  ``tv="autograd"`` takes the TV gradient by autograd."""
+    adaptive rules for alpha and beta.  ``tv="autograd"`` takes the TV gradient by autograd.
+    As in the reference, the TV steps are not followed by a non-negativity projection."""
     geom.check_supported()
     a1, b1 = normalisers(geom, "matched" if backprojector == "matched" else "ref", proj0.device, eps)
     bp = (lambda r: adjoint_batch(r, geom)) if backprojector == "matched" else (lambda r: _bp_ref(r, geom))
     tvg = tv_gradient_autograd if tv == "autograd" else tv_gradient
     x = (torch.zeros((proj0.shape[0],) + geom.vol_shape if proj0.dim() == 4 else geom.vol_shape,
                      device=proj0.device) if x0 is None else x0.detach().clone())
+    # a batch holds independent reconstructions: distances and alpha are per volume
+    dims = tuple(range(x.dim() - 3, x.dim()))
+    dist = lambda a, b: torch.sqrt(torch.sum((a - b) ** 2, dim=dims, keepdim=True))  # noqa: E731
+    alpha = torch.full_like(x[..., :1, :1, :1], alpha)
     for it in range(n_outer):
         f0 = x.clone()
         for _ in range(na):
@@ -183,14 +186,13 @@ def asd_pocs(proj0: torch.Tensor, geom: ScanGeometry, n_outer: int = 20, na: int
             upd = bp(r)
             upd = torch.where(b1 == 0, torch.zeros_like(upd), upd)
             x = torch.clamp_min(x + beta * upd / (b1 + eps), 0.0)
-        dp = math.sqrt(float(torch.sum((x - f0) ** 2)))
+        dp = dist(x, f0)
         dtvg = alpha * dp
         f0 = x.clone()
         for _ in range(ng):
             x = x - dtvg * tvg(x, eps)
-        dg = math.sqrt(float(torch.sum((x - f0) ** 2)))
-        if dg > r_max * dp:
-            alpha *= alpha_red
+        dg = dist(x, f0)
+        alpha = torch.where(dg > r_max * dp, alpha * alpha_red, alpha)
         beta *= beta_red
         if callback is not None:
             callback(it, x)

@@ -164,23 +164,35 @@ def test_sart_50_iterations_through_autograd_on_batch(dev):
     assert torch.isfinite(x).all()
 
 
+def _tv(x):
+    d = torch.sqrt(1e-7 + (x[..., 1:, 1:, 1:] - x[..., :-1, 1:, 1:]) ** 2
+                   + (x[..., 1:, 1:, 1:] - x[..., 1:, :-1, 1:]) ** 2
+                   + (x[..., 1:, 1:, 1:] - x[..., 1:, 1:, :-1]) ** 2)
+    return torch.sum(d, dim=(-3, -2, -1))
+
+
 def test_asd_pocs_tv_steps_through_autograd(dev):
     geom = ctr.ScanGeometry(make_params(**dict(C4, NumberOfViews=60)))
     vols = torch.stack([torch.clamp_min(phantom.ellipsoid_volume(geom, phantom.random_ellipsoids(
         8, phantom.half_extent(geom), 420 + b), device=dev), 0.0) for b in range(2)])
     from ctreconstruction_amd.autograd import forward_batch
     proj = forward_batch(vols, geom)
-    tvs = []
-
-    def cb(it, x):
-        tvs.append(float(torch.sum(torch.sqrt(1e-7 + (x[..., 1:, 1:, 1:] - x[..., :-1, 1:, 1:]) ** 2
-                                              + (x[..., 1:, 1:, 1:] - x[..., 1:, :-1, 1:]) ** 2
-                                              + (x[..., 1:, 1:, 1:] - x[..., 1:, 1:, :-1]) ** 2))))
-
-    x = IT.asd_pocs(proj, geom, n_outer=2, na=5, ng=20, backprojector="matched", tv="autograd", callback=cb)
+    kw = dict(na=5, backprojector="matched")
+    # one outer iteration: the 20 TV steps lower the TV of the POCS result of every volume
+    x_pocs = IT.asd_pocs(proj, geom, n_outer=1, ng=0, **kw)
+    x_tv = IT.asd_pocs(proj, geom, n_outer=1, ng=20, tv="autograd", **kw)
+    x_tv_ref = IT.asd_pocs(proj, geom, n_outer=1, ng=20, tv="reference", **kw)
+    tv0, tv1 = _tv(x_pocs), _tv(x_tv)
+    print(f"ASD-POCS TV per volume: POCS {tv0.tolist()} -> after 20 TV steps {tv1.tolist()}")
+    assert (tv1 < tv0).all()
+    scale = x_tv_ref.abs().max()
+    assert (x_tv - x_tv_ref).abs().max() <= 1e-5 * scale
+    # two outer iterations against plain non-negative SART
+    x = IT.asd_pocs(proj, geom, n_outer=2, ng=20, tv="autograd", **kw)
     rmse = float(torch.sqrt(torch.mean((x - vols) ** 2)))
     x_sart = IT.sart(proj, geom, 10, relaxation=1.0, nonneg=True)
     rmse_sart = float(torch.sqrt(torch.mean((x_sart - vols) ** 2)))
-    print(f"ASD-POCS rmse {rmse:.4f} (10-step SART {rmse_sart:.4f}), TV {tvs}")
-    assert torch.isfinite(x).all() and (x >= 0).all()
+    print(f"ASD-POCS rmse {rmse:.4f} (10-step SART {rmse_sart:.4f})")
+    # the TV steps are not followed by a positivity projection (python/ASD_POCS.py:186-188)
+    assert torch.isfinite(x).all() and x.min() >= -0.01 * x.max()
     assert rmse < 1.5 * rmse_sart
