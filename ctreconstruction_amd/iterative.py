@@ -85,7 +85,9 @@ def sart(proj0: torch.Tensor, geom: ScanGeometry, n_iter: int, relaxation: float
     """
     geom.check_supported()
     a1, b1 = normalisers(geom, backprojector, proj0.device, eps)
-    w = 1.0 / (a1 + eps)
+    # rays that miss the volume carry no information (their measured value is noise); they are
+    # left out as in python/ASD_POCS.py:175, instead of being amplified by 1 / eps
+    w = torch.where(a1 > 0, 1.0 / (a1 + eps), torch.zeros_like(a1))
     x = (torch.zeros((proj0.shape[0],) + geom.vol_shape if proj0.dim() == 4 else geom.vol_shape,
                      device=proj0.device) if x0 is None else x0.detach().clone())
     for it in range(n_iter):
@@ -167,13 +169,18 @@ def asd_pocs(proj0: torch.Tensor, geom: ScanGeometry, n_outer: int = 20, na: int
     """ASD-POCS (python/ASD_POCS.py:130-202): ``na`` non-negative SART steps, then ``ng``
     normalised TV steepest-descent steps of size alpha * |dx_POCS|, with the reference's
     adaptive rules for alpha and beta.  ``tv="autograd"`` takes the TV gradient by autograd.
-    As in the reference, the TV steps are not followed by a non-negativity projection."""
+    ``x0=None`` starts from the FDK reconstruction, as the reference script does.  As in the
+    reference, the TV steps are not followed by a non-negativity projection."""
     geom.check_supported()
     a1, b1 = normalisers(geom, "matched" if backprojector == "matched" else "ref", proj0.device, eps)
     bp = (lambda r: adjoint_batch(r, geom)) if backprojector == "matched" else (lambda r: _bp_ref(r, geom))
     tvg = tv_gradient_autograd if tv == "autograd" else tv_gradient
-    x = (torch.zeros((proj0.shape[0],) + geom.vol_shape if proj0.dim() == 4 else geom.vol_shape,
-                     device=proj0.device) if x0 is None else x0.detach().clone())
+    if x0 is None:
+        # the reference starts from the FDK reconstruction (python/ASD_POCS.py:142-143)
+        x = (torch.stack([P.fdk(p.contiguous(), geom) for p in proj0]) if proj0.dim() == 4
+             else P.fdk(proj0.contiguous(), geom))
+    else:
+        x = x0.detach().clone()
     # a batch holds independent reconstructions: distances and alpha are per volume
     dims = tuple(range(x.dim() - 3, x.dim()))
     dist = lambda a, b: torch.sqrt(torch.sum((a - b) ** 2, dim=dims, keepdim=True))  # noqa: E731

@@ -4,7 +4,7 @@ Scan (the sparse-view benchmark): 256^3 volume at 0.5 mm of seeded random ellips
 mu_water = 0.02/mm, 384 x 384 flat detector at 1.0 mm, SID 1000, SDD 1500, Poisson noise at
 1e5 photons.  A training pair is made by the method itself from one phantom:
 
-* input  = 20-iteration SART of the 60-view noisy scan,
+* input  = 20-iteration SART (relaxation 0.5, matched backprojector) of the 60-view noisy scan,
 * target = FDK of the 720-view noisy scan (Shepp-Logan window).
 
 The network (``prior.ResidualPrior``) sees 3 adjacent slices of the input and predicts the
@@ -45,7 +45,7 @@ class PairFactory:
 
     def __init__(self, device="cuda", n: int = 256, sparse_views: int = 60, full_views: int = 720,
                  sart_iters: int = 20, relaxation: float = 0.5, photons: float = 1e5,
-                 n_ellipsoids: int = 20, backprojector: str = "ref"):
+                 n_ellipsoids: int = 20, backprojector: str = "matched"):
         self.device = torch.device(device)
         self.sparse = ScanGeometry(scan_params(sparse_views, n))
         self.full = ScanGeometry(scan_params(full_views, n))
@@ -81,17 +81,26 @@ class PairFactory:
 
 
 def train(net: PR.ResidualPrior, factory: PairFactory, n_pairs: int, steps_per_pair: int = 8,
-          batch: int = 16, crop: int = 128, lr: float = 1e-3, seed: int = 0, log=None):
-    """Train on pairs generated on the fly (seeds seed .. seed + 3 n_pairs); returns the loss log."""
-    opt = torch.optim.Adam(net.parameters(), lr=lr)
-    rng = np.random.default_rng(seed)
+          batch: int = 16, crop: int = 128, lr: float = 1e-3, seed: int = 0, log=None,
+          opt: torch.optim.Optimizer | None = None, start_pair: int = 0, end_pair: int | None = None):
+    """Train on pairs generated on the fly: pair k comes from phantom seed ``seed + 3 k``.
+
+    The learning rate follows one cosine schedule over all ``n_pairs * steps_per_pair`` steps;
+    pairs ``start_pair .. end_pair - 1`` are run, so that training can be split over several
+    processes (pass the returned optimizer state back in).  Crop positions and slices of pair k
+    come from a generator seeded with (seed, k), so a split run draws what one run would.
+    Returns (optimizer, losses)."""
+    if opt is None:
+        opt = torch.optim.Adam(net.parameters(), lr=lr)
+    end_pair = n_pairs if end_pair is None else min(end_pair, n_pairs)
     losses = []
     total = n_pairs * steps_per_pair
-    step = 0
-    for k in range(n_pairs):
+    for k in range(start_pair, end_pair):
+        rng = np.random.default_rng([seed, k])
         x_in, target, _ = factory.pair(seed + 3 * k)
         nz, ny, nx = x_in.shape
-        for _ in range(steps_per_pair):
+        for j in range(steps_per_pair):
+            step = k * steps_per_pair + j
             lr_now = lr * 0.5 * (1 + math.cos(math.pi * step / max(total, 1)))
             for gp in opt.param_groups:
                 gp["lr"] = lr_now
@@ -105,11 +114,10 @@ def train(net: PR.ResidualPrior, factory: PairFactory, n_pairs: int, steps_per_p
             opt.zero_grad(set_to_none=True)
             loss.backward()
             opt.step()
-            losses.append(float(loss))
-            step += 1
+            losses.append(float(loss.detach()))
         if log is not None:
             log(k, float(np.mean(losses[-steps_per_pair:])))
-    return losses
+    return opt, losses
 
 
 def residual(geom: ScanGeometry, x: torch.Tensor, proj: torch.Tensor) -> float:
@@ -136,11 +144,13 @@ def learned_reconstruction(factory: PairFactory, net: PR.ResidualPrior, proj: to
 
 
 def time_to_rmse(factory: PairFactory, net: PR.ResidualPrior, seed: int, plain_iters: int = 200):
-    """Wall time for plain SART to reach its ``plain_iters``-iteration RMSE vs the learned pipeline.
+    """Wall time of plain SART (``plain_iters`` iterations) against the learned pipeline, and
+    the RMSE (vs the phantom) of both.
 
-    Plain SART runs twice: once with a per-iteration RMSE callback (untimed) to find the first
-    iteration at the final RMSE, once untimed-callback-free for the wall time, which is scaled by
-    that iteration's share (every iteration does the same work)."""
+    Plain SART runs twice: once with a per-iteration RMSE callback (untimed) to record its RMSE
+    curve, once without callback for the wall time.  ``speedup`` compares with the full
+    ``plain_iters`` run; ``speedup_to_rmse`` with the share of it up to the first iteration
+    whose RMSE is already the final one (every iteration does the same work)."""
     vol = factory.phantom(seed)
     proj = factory.sparse_scan(vol, seed + 1)
     g = factory.sparse
@@ -154,18 +164,22 @@ def time_to_rmse(factory: PairFactory, net: PR.ResidualPrior, seed: int, plain_i
     t0 = time.perf_counter()
     IT.sart(proj, g, plain_iters, relaxation=factory.relaxation, backprojector=factory.backprojector)
     torch.cuda.synchronize()
-    t_plain = (time.perf_counter() - t0) * it_target / plain_iters
+    t_plain = time.perf_counter() - t0
     learned_reconstruction(factory, net, proj)  # warm-up: kernel loading is not timed
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     x_l, info = learned_reconstruction(factory, net, proj)
     torch.cuda.synchronize()
     t_learned = time.perf_counter() - t0
-    return {"plain_iters": plain_iters, "plain_rmse": target, "plain_iters_to_rmse": it_target,
-            "plain_time_s": t_plain, "plain_rmse_at_20": errs[min(19, len(errs) - 1)],
-            "learned_rmse": rmse(x_l), "learned_time_s": t_learned, "speedup": t_plain / t_learned,
-            "learned_reaches_plain_rmse": rmse(x_l) <= target,
-            "residual_plain": residual(g, x_plain, proj), **info}
+    r_l = rmse(x_l)
+    return {"plain_iters": plain_iters, "plain_rmse": target, "plain_time_s": t_plain,
+            "plain_iters_to_rmse": it_target, "plain_time_to_rmse_s": t_plain * it_target / plain_iters,
+            "learned_rmse": r_l, "learned_time_s": t_learned, "learned_reaches_plain_rmse": r_l <= target,
+            "speedup": t_plain / t_learned,
+            "speedup_to_rmse": t_plain * it_target / plain_iters / t_learned,
+            "residual_plain": residual(g, x_plain, proj),
+            "plain_rmse_curve": {i: errs[i - 1] for i in (1, 5, 10, 20, 25, 50, 100, 150, 200) if i <= len(errs)},
+            **info}
 
 
 def save_weights(net: PR.ResidualPrior, path: str, meta: dict | None = None) -> None:
@@ -181,8 +195,23 @@ def load_weights(path: str, device="cuda") -> PR.ResidualPrior:
     return net.to(device)
 
 
+def save_checkpoint(net: PR.ResidualPrior, opt: torch.optim.Optimizer, next_pair: int, path: str) -> None:
+    torch.save({"net": {k: v.detach().cpu() for k, v in net.state_dict().items()},
+                "opt": opt.state_dict(), "next_pair": next_pair}, path)
+
+
+def load_checkpoint(path: str, lr: float, device="cuda"):
+    ck = torch.load(path, map_location="cpu", weights_only=True)
+    net = PR.ResidualPrior().to(device)
+    net.load_state_dict(ck["net"])
+    opt = torch.optim.Adam(net.parameters(), lr=lr)
+    opt.load_state_dict(ck["opt"])
+    return net, opt, int(ck["next_pair"])
+
+
 def main(argv=None):
-    """Train the prior on pairs generated on the fly and evaluate time-to-RMSE on a held-out phantom."""
+    """Train the prior on pairs generated on the fly and evaluate time-to-RMSE on a held-out
+    phantom.  ``--until`` and ``--resume`` split the training over several processes."""
     import argparse
     import json
 
@@ -193,20 +222,33 @@ def main(argv=None):
     ap.add_argument("--crop", type=int, default=128)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--out", default="prior.safetensors")
+    ap.add_argument("--checkpoint", default=None, help="write a resumable checkpoint here")
+    ap.add_argument("--resume", default=None, help="continue from this checkpoint")
+    ap.add_argument("--until", type=int, default=None, help="stop before this pair index")
     ap.add_argument("--eval-seed", type=int, default=10000)
-    ap.add_argument("--backprojector", default="ref", choices=["ref", "edge", "matched"])
+    ap.add_argument("--no-eval", action="store_true")
+    ap.add_argument("--backprojector", default="matched", choices=["ref", "edge", "matched"])
     args = ap.parse_args(argv)
     factory = PairFactory(backprojector=args.backprojector)
-    net = PR.ResidualPrior(seed=0).cuda()
+    if args.resume:
+        net, opt, start = load_checkpoint(args.resume, args.lr)
+    else:
+        net, opt, start = PR.ResidualPrior(seed=0).cuda(), None, 0
     t0 = time.perf_counter()
     log = lambda k, l: print(json.dumps({"pair": k, "loss": l, "t": time.perf_counter() - t0}), flush=True)  # noqa: E731
-    train(net, factory, args.pairs, args.steps_per_pair, args.batch, args.crop, args.lr, seed=1, log=log)
+    opt, _ = train(net, factory, args.pairs, args.steps_per_pair, args.batch, args.crop, args.lr, seed=1,
+                   log=log, opt=opt, start_pair=start, end_pair=args.until)
     t_train = time.perf_counter() - t0
-    save_weights(net, args.out, {"pairs": args.pairs, "steps_per_pair": args.steps_per_pair,
-                                 "train_s": t_train, "backprojector": args.backprojector})
-    res = time_to_rmse(factory, net, args.eval_seed)
-    res["train_s"] = t_train
-    print(json.dumps(res), flush=True)
+    end = args.pairs if args.until is None else min(args.until, args.pairs)
+    if args.checkpoint:
+        save_checkpoint(net, opt, end, args.checkpoint)
+    save_weights(net, args.out, {"pairs": end, "steps_per_pair": args.steps_per_pair,
+                                 "batch": args.batch, "crop": args.crop, "lr": args.lr,
+                                 "backprojector": args.backprojector})
+    print(json.dumps({"trained_pairs": end, "train_s_this_run": t_train}), flush=True)
+    if not args.no_eval:
+        res = time_to_rmse(factory, net, args.eval_seed)
+        print(json.dumps(res), flush=True)
 
 
 if __name__ == "__main__":

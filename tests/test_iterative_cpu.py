@@ -56,3 +56,24 @@ def test_flip_transpose_is_the_adjoint_of_the_convolution():
     lhs = torch.sum(conv(x, w) * y)
     rhs = torch.sum(x * conv(y, wt))
     assert abs(float(lhs - rhs)) < 1e-10 * abs(float(lhs))
+
+
+def test_training_checkpoint_round_trip(tmp_path):
+    from ctreconstruction_amd import learned as L
+    net = PR.ResidualPrior(seed=2)
+    opt = torch.optim.Adam(net.parameters(), lr=1e-3)
+    g = torch.Generator().manual_seed(0)
+    for p in net.parameters():
+        p.grad = torch.randn(p.shape, generator=g)
+    opt.step()
+    path = str(tmp_path / "ck.pt")
+    L.save_checkpoint(net, opt, 17, path)
+    net2, opt2, nxt = L.load_checkpoint(path, 1e-3, device="cpu")
+    assert nxt == 17
+    for a, b in zip(net.parameters(), net2.parameters()):
+        assert torch.equal(a.detach(), b.detach())
+    s1, s2 = opt.state_dict()["state"], opt2.state_dict()["state"]
+    assert s1.keys() == s2.keys()
+    for k in s1:
+        assert torch.equal(s1[k]["exp_avg"], s2[k]["exp_avg"])
+        assert torch.equal(s1[k]["exp_avg_sq"], s2[k]["exp_avg_sq"])
