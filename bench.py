@@ -28,6 +28,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+BP_KERNEL = "dd_backproject_tile_kernel<8>"  # the dominant kernel of one FDK step
 
 CONFIGS = {
     "c3": dict(n=512, voxel=0.25, views=720, nu=1024, nv=768, pitch=0.4, ellipsoids=40, seed=3,
@@ -50,14 +51,16 @@ def make_params(cfg, filter_type="shepp-logan"):
     }
 
 
-def traffic_from_profile(name):
+def traffic_from_profile(name, kernel):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled by the
-    copy-kernel calibration), recorded in profiles/<name>.json; None when absent."""
+    copy-kernel calibration), recorded in profiles/<name>.json; None when absent or when the
+    record is for another kernel."""
     path = os.path.join(ROOT, "profiles", name + ".json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("traffic_bytes")
+        rec = json.load(f)
+    return rec.get("traffic_bytes") if rec.get("kernel") == kernel else None
 
 
 def cpu_baseline(params, q_gpu, geom, dev, sample_views=(0, 1), slab=8):
@@ -106,9 +109,9 @@ WEIGHTS = os.path.join(ROOT, "ctreconstruction_amd", "weights", "prior_sparse60.
 
 
 def learned_vs_sart(seed=10000):
-    """Sparse-view benchmark (256^3, 60 views, 384^2, Poisson 1e5): wall time for plain SART to
-    reach its 200-iteration RMSE vs the learned prior + SART pipeline (trained weights in
-    ctreconstruction_amd/weights/); None when no trained weights are committed."""
+    """Sparse-view benchmark (256^3, 60 views, 384^2, Poisson 1e5, held-out phantom): wall time
+    and RMSE vs the phantom of plain 200-iteration SART and of the learned prior + SART pipeline
+    (trained weights in ctreconstruction_amd/weights/); None when no trained weights exist."""
     if not os.path.exists(WEIGHTS):
         return None
     from ctreconstruction_amd import learned as L
@@ -217,10 +220,10 @@ def main(argv=None):
                    "voxel_mm": cfg["voxel"], "views": geom.nview, "detector": [geom.nv, geom.nu],
                    "pixel_mm": cfg["pitch"], "sad_mm": geom.sad, "sdd_mm": geom.sdd,
                    "filter": geom.filter_type, "step": "fdk_filter + dd_backproject (mode ref)"},
-        "roofline": {"bound": "hbm", "kernel": "dd_backproject_tile_kernel<8>", "achieved": achieved,
+        "roofline": {"bound": "hbm", "kernel": BP_KERNEL, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "algorithmic_bytes": bp_bytes,
-                     "traffic": traffic_from_profile(f"dd_backproject_{args.config}_traffic")},
+                     "traffic": traffic_from_profile(f"dd_backproject_{args.config}_traffic", BP_KERNEL)},
         "fdk_backprojection": {"s": bp_s, "gups": updates / bp_s / 1e9},
         "fdk_filter": {"s": sec - bp_s},
         "dd_forward": {"s": fp_s, "gups": updates / fp_s / 1e9, "ray_slab_updates_per_s": ray_slab / fp_s},

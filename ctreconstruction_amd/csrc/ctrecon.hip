@@ -44,8 +44,10 @@ int check_geometry(const ctr_geometry* g) {
     const double vox = (double)g->nx * g->ny * g->nz;
     const double cells = (double)g->n_views * g->nv * g->nu;
     if (vox >= 2147483647.0 || cells >= 2147483647.0) return CTR_ERR_INVALID;
-    // grid y/z limits: transposes (nz/32, ny), backprojection/adjoint (ny/4, nz/4)
-    if (g->n_views > 65535 || g->nv > 65535 * 4 || g->ny > 65535 || g->nz > 65535)
+    // grid y/z limits: transposes (nz/32, ny), backprojection/adjoint (ny/4, nz/4); the
+    // forward projector packs transverse cell indices into 24 bits
+    if (g->n_views > 65535 || g->nv > 65535 * 4 || g->ny > 65535 || g->nz > 65535 ||
+        g->nx >= (1 << 24))
         return CTR_ERR_INVALID;
     return CTR_OK;
 }
@@ -350,10 +352,19 @@ __global__ void __launch_bounds__(256) zfast_transpose_kernel(const float* __res
 // DD forward projection with one wavefront per (view, detector column, 64 rows).  All rays of a
 // column share their transverse footprint at every slab (the U-edge lines depend on the column
 // only), so the transverse overlaps, the slab test and the transverse addresses are uniform
-// across the wavefront and computed once for its 64 rays; each lane adds its row's z footprint.
-// The volume is read from a z-fastest copy, so the lanes' taps are consecutive in memory.
-// Same semantics and weights as dd_forward_kernel (python/GPUFuncs_pycuda.py:398-524).
+// across the wavefront: lane L works them out for slab n0 + L of a batch of 64 slabs, and the
+// wavefront then walks the batch reading them back from lane q as scalars.  Each lane adds its
+// row's z footprint, c = nu * tt(n) + gamma for the row's two edge slopes nu, whose width
+// |nu1 - nu2| |tt(n)| factors into a per-lane constant and a per-slab scalar, so no per-lane
+// reciprocal is taken per slab.  The volume is read from a z-fastest copy, so the lanes' taps
+// are consecutive in memory.  Same semantics and weights as dd_forward_kernel
+// (python/GPUFuncs_pycuda.py:398-524): overlaps below zero_tol times the footprint width are
+// dropped, a single-cell footprint has weight 1.
 constexpr int kColWaves = 4;
+
+__device__ inline float lane_f(float v, int q) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), q));
+}
 
 __global__ void __launch_bounds__(256) dd_forward_column_kernel(
     const float* __restrict__ v_ys, const float* __restrict__ v_xs, float* __restrict__ proj,
@@ -366,7 +377,9 @@ __global__ void __launch_bounds__(256) dd_forward_column_kernel(
     const int view = t % n_views;
     const int rest = t / n_views;
     const int cg = rest % col_groups, rb = rest / col_groups;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    // wave-uniform: lets the compiler keep column data and addresses in scalar registers
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     const int j = cg * kColWaves + wave;
     if (j >= nu) return;
     const int k = rb * 64 + lane;
@@ -392,81 +405,109 @@ __global__ void __launch_bounds__(256) dd_forward_column_kernel(
     const float nu1 = rows[kk + 1], nu2 = rows[kk];
     const float vc = rows[nv + 1 + kk];
     const float s2 = vc * inv;
-    const float Lk = sec * sqrtf(1.0f + s2 * s2);
-    const float tol = zero_tol;
+    const float dnu = fabsf(nu1 - nu2);  // z footprint width per unit |tt|
+    const float Lk = sec * sqrtf(1.0f + s2 * s2) / dnu;
+    const float tol_c = zero_tol * dnu;
+    const float fnz = (float)nz, fnt = (float)nt;
     float acc = 0.0f;
-    for (int n = 0; n < ns; ++n) {
-        const float fn = (float)n;
-        // transverse footprint: uniform across the wavefront
-        const float a1 = fmaf(a_be, fn, a_al);
-        const float a2 = fmaf(b_be, fn, b_al);
-        const float alo = fminf(a1, a2), ahi = fmaxf(a1, a2);
-        const int kas = max((int)floorf(alo), 0), kae = min((int)floorf(ahi), nt - 1);
-        if (kas > kae) continue;
-        const float wa = ahi - alo;
-        // z footprint of this lane's row
-        const float tt = fmaf(dT, fn, T0);
-        const float c1 = fmaf(nu1, tt, gam);
-        const float c2 = fmaf(nu2, tt, gam);
-        const float clo = fminf(c1, c2), chi = fmaxf(c1, c2);
-        const int kcs = max((int)floorf(clo), 0), kce = min((int)floorf(chi), nz - 1);
-        if (kcs > kce) continue;
-        const float wc = chi - clo;
-        const float* slab = src + (size_t)n * slab_stride;
-        float sum = 0.0f;
-        if (kae - kas <= 2 && kce - kcs <= 2) {
-            float oa[3], oc[3];
-            {
-                const float lo = fmaxf(alo, 0.0f), hi = fminf(ahi, (float)nt);
-                const float s1 = floorf(lo) + 1.0f;
-                oa[0] = fminf(hi, s1) - lo;
-                oa[1] = fminf(hi, s1 + 1.0f) - s1;
-                oa[2] = hi - (s1 + 1.0f);
-                const float tl = tol * wa;
-#pragma unroll
-                for (int i = 0; i < 3; ++i) oa[i] = (oa[i] >= tl) ? oa[i] : 0.0f;
-            }
-            {
-                const float lo = fmaxf(clo, 0.0f), hi = fminf(chi, (float)nz);
-                const float s1 = floorf(lo) + 1.0f;
-                oc[0] = fminf(hi, s1) - lo;
-                oc[1] = fminf(hi, s1 + 1.0f) - s1;
-                oc[2] = hi - (s1 + 1.0f);
-                const float tl = tol * wc;
-#pragma unroll
-                for (int i = 0; i < 3; ++i) oc[i] = (oc[i] >= tl) ? oc[i] : 0.0f;
-            }
-            const unsigned da = (unsigned)(kae - kas), dc = (unsigned)(kce - kcs);
-            const unsigned nz4 = 4u * (unsigned)nz;
-            const unsigned base = 4u * ((unsigned)kas * (unsigned)nz + (unsigned)kcs);
-            const unsigned ua[3] = {0u, nz4 * min(1u, da), nz4 * min(2u, da)};
-            const unsigned uc[3] = {0u, 4u * min(1u, dc), 4u * min(2u, dc)};
-            const char* sb = reinterpret_cast<const char*>(slab);
-            float f[3][3];
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int jz = 0; jz < 3; ++jz)
-                    f[i][jz] = *reinterpret_cast<const float*>(sb + (base + ua[i] + uc[jz]));
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                const float col = fmaf(oc[0], f[i][0], fmaf(oc[1], f[i][1], oc[2] * f[i][2]));
-                sum = fmaf(oa[i], col, sum);
-            }
-        } else {
-            for (int ka = kas; ka <= kae; ++ka) {
-                const float o = fminf(ahi, (float)(ka + 1)) - fmaxf(alo, (float)ka);
-                if (o < tol * wa) continue;
-                const float* line = slab + (size_t)ka * nz;
-                float col = 0.0f;
-                for (int kc = kcs; kc <= kce; ++kc) {
-                    const float p = fminf(chi, (float)(kc + 1)) - fmaxf(clo, (float)kc);
-                    if (p >= tol * wc) col = fmaf(line[kc], p, col);
+    for (int n0 = 0; n0 < ns; n0 += 64) {
+        // lane L: transverse footprint of slab n0 + L.  Mode 0: misses the volume, 1: at most
+        // three cells (weights pre-divided by wa |tt|), 2: wider, explicit loop.
+        float b_tt = 1.0f, b_w0 = 0.0f, b_w1 = 0.0f, b_w2 = 0.0f, b_alo = 0.0f, b_ahi = 0.0f;
+        int b_pack = 0;
+        {
+            const int n = n0 + lane;
+            if (n < ns) {
+                const float fn = (float)n;
+                const float a1 = fmaf(a_be, fn, a_al), a2 = fmaf(b_be, fn, b_al);
+                const float alo = fminf(a1, a2), ahi = fmaxf(a1, a2);
+                const int kas = max((int)floorf(alo), 0), kae = min((int)floorf(ahi), nt - 1);
+                b_tt = fmaf(dT, fn, T0);
+                b_alo = alo;
+                b_ahi = ahi;
+                if (kas <= kae) {
+                    const int da = kae - kas;
+                    const float wa = ahi - alo;
+                    // overlaps with cells kas .. kas+2 (floor(max(alo, 0)) == kas); negative
+                    // values past the footprint fail the threshold test
+                    const float lo = fmaxf(alo, 0.0f), hi = fminf(ahi, fnt);
+                    const float s1 = (float)kas + 1.0f;
+                    float o0 = fminf(hi, s1) - lo, o1 = fminf(hi, s1 + 1.0f) - s1, o2 = hi - (s1 + 1.0f);
+                    const float tl = zero_tol * wa;
+                    o0 = (o0 >= tl) ? o0 : 0.0f;
+                    o1 = (o1 >= tl) ? o1 : 0.0f;
+                    o2 = (o2 >= tl) ? o2 : 0.0f;
+                    const float r = __builtin_amdgcn_rcpf(wa * fabsf(b_tt));
+                    b_w0 = o0 * r;
+                    b_w1 = o1 * r;
+                    b_w2 = o2 * r;
+                    b_pack = (da <= 2) ? ((1 << 28) | (da << 24) | kas) : (2 << 28);
                 }
-                sum = fmaf(col, o, sum);
             }
         }
-        acc = fmaf(sum, __builtin_amdgcn_rcpf(wa * wc), acc);
+        const int nb = min(64, ns - n0);
+        for (int q = 0; q < nb; ++q) {
+            const int pack = __builtin_amdgcn_readlane(b_pack, q);
+            const int mode = (pack >> 28) & 3;
+            if (mode == 0) continue;
+            const float tt = lane_f(b_tt, q);
+            const float att = fabsf(tt);
+            const float* slab = src + (size_t)(n0 + q) * slab_stride;
+            const float c1 = fmaf(nu1, tt, gam), c2 = fmaf(nu2, tt, gam);
+            const float clo = fminf(c1, c2), chi = fmaxf(c1, c2);
+            const int kcs = max((int)floorf(clo), 0), kce = min((int)floorf(chi), nz - 1);
+            const float tl = tol_c * att;  // zero_tol * wc
+            if (mode == 1 && !__any(kce - kcs > 2)) {
+                const int kas = pack & 0xffffff, da = (pack >> 24) & 15;
+                const float lo = fmaxf(clo, 0.0f), hi = fminf(chi, fnz);
+                const float s1 = (float)kcs + 1.0f;
+                float o0 = fminf(hi, s1) - lo, o1 = fminf(hi, s1 + 1.0f) - s1, o2 = hi - (s1 + 1.0f);
+                o0 = (o0 >= tl) ? o0 : 0.0f;
+                o1 = (o1 >= tl) ? o1 : 0.0f;
+                o2 = (o2 >= tl) ? o2 : 0.0f;
+                // taps past the footprint repeat its last cell (weight 0); a footprint outside
+                // the volume reads cell nz - 1 with all weights 0
+                const int zc = min(kcs, nz - 1);
+                const int dc = max(kce - zc, 0);
+                const unsigned v0 = 4u * (unsigned)zc;
+                const unsigned v1 = v0 + 4u * (unsigned)min(1, dc), v2 = v0 + 4u * (unsigned)min(2, dc);
+                const float w0 = lane_f(b_w0, q), w1 = lane_f(b_w1, q), w2 = lane_f(b_w2, q);
+                const char* r0 = reinterpret_cast<const char*>(slab + (size_t)kas * nz);
+                const char* r1 = reinterpret_cast<const char*>(slab + (size_t)(kas + min(1, da)) * nz);
+                const char* r2 = reinterpret_cast<const char*>(slab + (size_t)(kas + min(2, da)) * nz);
+                const float f00 = *reinterpret_cast<const float*>(r0 + v0);
+                const float f01 = *reinterpret_cast<const float*>(r0 + v1);
+                const float f02 = *reinterpret_cast<const float*>(r0 + v2);
+                const float f10 = *reinterpret_cast<const float*>(r1 + v0);
+                const float f11 = *reinterpret_cast<const float*>(r1 + v1);
+                const float f12 = *reinterpret_cast<const float*>(r1 + v2);
+                const float f20 = *reinterpret_cast<const float*>(r2 + v0);
+                const float f21 = *reinterpret_cast<const float*>(r2 + v1);
+                const float f22 = *reinterpret_cast<const float*>(r2 + v2);
+                const float col0 = fmaf(o0, f00, fmaf(o1, f01, o2 * f02));
+                const float col1 = fmaf(o0, f10, fmaf(o1, f11, o2 * f12));
+                const float col2 = fmaf(o0, f20, fmaf(o1, f21, o2 * f22));
+                acc = fmaf(w0, col0, fmaf(w1, col1, fmaf(w2, col2, acc)));
+            } else {
+                const float alo = lane_f(b_alo, q), ahi = lane_f(b_ahi, q);
+                const float wa = ahi - alo;
+                const int kas = max((int)floorf(alo), 0), kae = min((int)floorf(ahi), nt - 1);
+                if (kcs > kce) continue;
+                float sum = 0.0f;
+                for (int ka = kas; ka <= kae; ++ka) {
+                    const float o = fminf(ahi, (float)(ka + 1)) - fmaxf(alo, (float)ka);
+                    if (o < zero_tol * wa) continue;
+                    const float* line = slab + (size_t)ka * nz;
+                    float col = 0.0f;
+                    for (int kc = kcs; kc <= kce; ++kc) {
+                        const float p = fminf(chi, (float)(kc + 1)) - fmaxf(clo, (float)kc);
+                        if (p >= tl) col = fmaf(line[kc], p, col);
+                    }
+                    sum = fmaf(col, o, sum);
+                }
+                acc = fmaf(sum, __builtin_amdgcn_rcpf(wa * att), acc);
+            }
+        }
     }
     if (active) proj[out_idx] = acc * Lk;
 }
@@ -736,24 +777,45 @@ __device__ inline void bp_column_view(const BpView& v, const float* __restrict__
     }
     const float dd = v.R - yr0;
     const float W = scale * (v.R * v.R) * __builtin_amdgcn_rcpf(dd * dd);
-    // v-footprints of the ZC voxels (index space)
-    float vlo[ZC], vhi[ZC], inv_v[ZC];
-    float rlo = 3.0e38f, rhi = -3.0e38f;
-#pragma unroll
-    for (int i = 0; i < ZC; ++i) {
-        const float e = v.Sz - fmaf((float)i, dz, Z0);
-        const float ep = e + hz, em = e - hz;
-        const float smx = ep * (ep >= 0.0f ? rmax : rmin);
-        const float smn = em * (em >= 0.0f ? rmin : rmax);
-        const float g1 = v.Av - v.Bv * smn, g2 = v.Av - v.Bv * smx;
-        vlo[i] = fminf(g1, g2);
-        vhi[i] = fmaxf(g1, g2);
-        inv_v[i] = __builtin_amdgcn_rcpf(vhi[i] - vlo[i]);
-        if (i < nz_left) {
-            rlo = fminf(rlo, vlo[i]);
-            rhi = fmaxf(rhi, vhi[i]);
-        }
+    // v-footprints (index space) of the ZC voxels.  Voxel i has e = Sz - z = e0 - i dz and its
+    // footprint ends are g1 = Av - Bv (e - hz) r1, g2 = Av - Bv (e + hz) r2 with r1 = rmin,
+    // r2 = rmax where e -/+ hz >= 0 and r1 = rmax, r2 = rmin where e -/+ hz <= 0.  When the whole
+    // chunk lies on one side of the source height (uniform across the workgroup) both ends are
+    // affine in i, and g2 - g1 = -Bv (smx - smn) has the sign of -Bv for every voxel.
+    const float e0 = v.Sz - Z0;
+    const float eL = e0 - (float)(ZC - 1) * dz;
+    const bool above = fminf(e0, eL) - hz >= 0.0f;
+    const bool lin = above || (fmaxf(e0, eL) + hz <= 0.0f);
+    float Alo, Dlo, Ahi, Dhi;
+    {
+        const float r1 = above ? rmin : rmax, r2 = above ? rmax : rmin;
+        const float A1 = v.Av - v.Bv * r1 * (e0 - hz), D1 = v.Bv * r1 * dz;
+        const float A2 = v.Av - v.Bv * r2 * (e0 + hz), D2 = v.Bv * r2 * dz;
+        const bool g1_high = v.Bv > 0.0f;
+        Alo = g1_high ? A2 : A1;
+        Dlo = g1_high ? D2 : D1;
+        Ahi = g1_high ? A1 : A2;
+        Dhi = g1_high ? D1 : D2;
     }
+    auto foot = [&](float fi, float& lo, float& hi) {
+        if (lin) {
+            lo = fmaf(fi, Dlo, Alo);
+            hi = fmaf(fi, Dhi, Ahi);
+        } else {
+            const float e = v.Sz - fmaf(fi, dz, Z0);
+            const float ep = e + hz, em = e - hz;
+            const float smx = ep * (ep >= 0.0f ? rmax : rmin);
+            const float smn = em * (em >= 0.0f ? rmin : rmax);
+            const float g1 = v.Av - v.Bv * smn, g2 = v.Av - v.Bv * smx;
+            lo = fminf(g1, g2);
+            hi = fmaxf(g1, g2);
+        }
+    };
+    // the ends move monotonically with i, so the chunk's row range comes from its end voxels
+    float lo0, hi0, loL, hiL;
+    foot(0.0f, lo0, hi0);
+    foot((float)(nz_left - 1), loL, hiL);
+    const float rlo = fminf(lo0, loL), rhi = fmaxf(hi0, hiL);
     const int ks = max(max((int)floorf(rlo), 0), row0);
     const int ke = min(min((int)floorf(rhi), nv - 1), row0 + nrows - 1);
     const int nr = ke - ks + 1;
@@ -761,7 +823,8 @@ __device__ inline void bp_column_view(const BpView& v, const float* __restrict__
     const int cbase = kus - col0;
     const int ntaps = kue - kus + 1;
     if (zero_tol <= 0.0f && nr <= kMaxRows) {
-        // row integral C at row boundaries ks .. ks+nr, in this work-item's LDS column
+        // row integral C at row boundaries ks .. ks+nr in this work-item's LDS column, plus a
+        // copy of the total at ks+nr+1 so that a lookup at the top needs no index clamp
         float csum = 0.0f;
         scan[0] = 0.0f;
         for (int j = 0; j < nr; ++j) {
@@ -774,22 +837,40 @@ __device__ inline void bp_column_view(const BpView& v, const float* __restrict__
             csum = fmaf(rk, W, csum);
             scan[(j + 1) * 256] = csum;
         }
-        // C(v) between row boundaries is linear; clamping v to [ks, ks + nr] gives C = 0 below
+        scan[(nr + 1) * 256] = csum;
+        // C(v) between row boundaries is linear; clamping v - ks to [0, nr] gives C = 0 below
         // and the total above (rows outside the detector or the window contribute nothing)
         const float fks = (float)ks, fnr = (float)nr;
+        auto integral = [&](float x) {
+            const float f = fminf(fmaxf(x, 0.0f), fnr);
+            const int jj = (int)f;
+            const float a = scan[jj * 256], b = scan[(jj + 1) * 256];
+            return fmaf(__builtin_amdgcn_fractf(f), b - a, a);
+        };
+        if (lin) {
+            const float alo = Alo - fks, ahi = Ahi - fks;
 #pragma unroll
-        for (int i = 0; i < ZC; ++i) {
-            float c2[2];
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const float f = fminf(fmaxf((h ? vhi[i] : vlo[i]) - fks, 0.0f), fnr);
-                const int jj = min((int)f, nr - 1);
-                const float a = scan[jj * 256], b = scan[(jj + 1) * 256];
-                c2[h] = fmaf(f - (float)jj, b - a, a);
+            for (int i = 0; i < ZC; ++i) {
+                const float lo = fmaf((float)i, Dlo, alo), hi = fmaf((float)i, Dhi, ahi);
+                acc[i] = fmaf(integral(hi) - integral(lo), __builtin_amdgcn_rcpf(hi - lo), acc[i]);
             }
-            acc[i] = fmaf(c2[1] - c2[0], inv_v[i], acc[i]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < ZC; ++i) {
+                float lo, hi;
+                foot((float)i, lo, hi);
+                lo -= fks;
+                hi -= fks;
+                acc[i] = fmaf(integral(hi) - integral(lo), __builtin_amdgcn_rcpf(hi - lo), acc[i]);
+            }
         }
         return;
+    }
+    float vlo[ZC], vhi[ZC], inv_v[ZC];
+#pragma unroll
+    for (int i = 0; i < ZC; ++i) {
+        foot((float)i, vlo[i], vhi[i]);
+        inv_v[i] = __builtin_amdgcn_rcpf(vhi[i] - vlo[i]);
     }
     for (int k = ks; k <= ke; ++k) {
         const float* row = src + (size_t)(k - row0) * stride + cbase;
@@ -816,7 +897,7 @@ __global__ void __launch_bounds__(256) dd_backproject_tile_kernel(
     float dx, float dy, float dz, int curved, float scale, float zero_tol, int accumulate,
     int tiles_x, int tiles_y, int tiles_z) {
     __shared__ float win[2][kWinCap];
-    __shared__ float scan_lds[(kMaxRows + 1) * 256];
+    __shared__ float scan_lds[(kMaxRows + 2) * 256];
     float* scan = scan_lds + threadIdx.x;
     // XCD-aware tile order: workgroup b runs on XCD b % 8; give each XCD a contiguous range
     const int T = tiles_x * tiles_y * tiles_z;
@@ -931,7 +1012,7 @@ __global__ void __launch_bounds__(256) dd_backproject_tile_kernel(
 // forward projector's own overlap weights from the same FP64-built tables, so a tap carries
 // y * L * o_a * o_c / (w_a * w_c) exactly as the forward sum carries it.
 constexpr int kAdjViewStride = 16;
-constexpr int kAdjZChunk = 4;
+constexpr int kAdjZChunk = 8;
 
 // Per-view constants (FP64 -> FP32):
 //   [0] branch  [1] Pt: source transverse index coordinate  [2] Ps: source slab coordinate
@@ -989,8 +1070,30 @@ __device__ inline float adj_edge_coord(const float* A, float tk, float n, int cu
     return (u2 - A[9]) * A[10];
 }
 
+// y'[view][k][j] = y[view][k][j] * L(j, k) / |nu_k+1 - nu_k|: the path-length factor and the
+// per-row part of the z normalisation of the forward projector, formed as it forms them.
+__global__ void __launch_bounds__(256) adj_prescale_kernel(const float* __restrict__ y,
+                                                           float* __restrict__ yl,
+                                                           const float* __restrict__ tabs,
+                                                           const float* __restrict__ rows, int nu,
+                                                           int nv, int n_cells, int stride) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n_cells) return;
+    const int j = c % nu;
+    const int r = c / nu;
+    const int k = r % nv, view = r / nv;
+    const float* T = tabs + (size_t)view * stride;
+    const float sec = T[4 * nu + 2 + j], inv = T[5 * nu + 2 + j];
+    const float s2 = rows[nv + 1 + k] * inv;
+    const float dnu = fabsf(rows[k + 1] - rows[k]);
+    yl[c] = y[c] * (sec * sqrtf(1.0f + s2 * s2) / dnu);
+}
+
+// One work-item per voxel column chunk of kAdjZChunk voxels.  Per view and candidate detector
+// column the rows that reach the chunk are walked once; each row's z footprint (the forward
+// projector's own fma of the same table values) is overlapped with every voxel of the chunk.
 __global__ void __launch_bounds__(256) dd_adjoint_kernel(
-    const float* __restrict__ y, float* __restrict__ vol, const float* __restrict__ tabs,
+    const float* __restrict__ yl, float* __restrict__ vol, const float* __restrict__ tabs,
     const float* __restrict__ rows, const float* __restrict__ at, int nx, int ny, int nz, int nu,
     int nv, int n_views, int stride, int curved, float scale, float zero_tol, int accumulate) {
     const int ix = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1002,6 +1105,7 @@ __global__ void __launch_bounds__(256) dd_adjoint_kernel(
 #pragma unroll
     for (int i = 0; i < kAdjZChunk; ++i) acc[i] = 0.0f;
     const float tol = zero_tol;
+    const float fz0 = (float)iz0, fz1 = (float)(iz0 + nzc);
     for (int view = 0; view < n_views; ++view) {
         const float* A = at + (size_t)view * kAdjViewStride;
         const int br = (int)A[0];
@@ -1017,7 +1121,7 @@ __global__ void __launch_bounds__(256) dd_adjoint_kernel(
         const int j1 = min((int)floorf(fmaxf(e1, e2) + 0.01f), nu - 1);
         const float* T = tabs + (size_t)view * stride;
         const float gam = T[6 * nu + 2];
-        const float* yv = y + (size_t)view * nv * nu;
+        const float* yv = yl + (size_t)view * nv * nu;
         for (int j = j0; j <= j1; ++j) {
             const float a1 = fmaf(T[nu + 1 + j], fn, T[j]);
             const float a2 = fmaf(T[nu + 2 + j], fn, T[j + 1]);
@@ -1026,29 +1130,36 @@ __global__ void __launch_bounds__(256) dd_adjoint_kernel(
             const float oa = fminf(ahi, fka + 1.0f) - fmaxf(alo, fka);
             if (!(oa > 0.0f) || oa < tol * wa) continue;
             const float t = fmaf(T[3 * nu + 2 + j], fn, T[2 * nu + 2 + j]);
-            const float sec = T[4 * nu + 2 + j], inv = T[5 * nu + 2 + j];
             const float rt = __builtin_amdgcn_rcpf(t);
-            for (int i = 0; i < nzc; ++i) {
-                const float fkc = (float)(iz0 + i);
-                // rows whose edges bracket [kc, kc+1]: nu_e = (c - gam) / t, e = (nu_e - nuA) / nuB
-                const float r1 = ((fkc - gam) * rt - A[11]) * A[12];
-                const float r2 = ((fkc + 1.0f - gam) * rt - A[11]) * A[12];
-                const int k0 = max((int)floorf(fminf(r1, r2) - 0.01f), 0);
-                const int k1 = min((int)floorf(fmaxf(r1, r2) + 0.01f), nv - 1);
-                float sum = 0.0f;
-                for (int k = k0; k <= k1; ++k) {
-                    const float c1 = fmaf(rows[k + 1], t, gam);
-                    const float c2 = fmaf(rows[k], t, gam);
-                    const float clo = fminf(c1, c2), chi = fmaxf(c1, c2);
-                    const float wc = chi - clo;
+            // normalisation as the forward projector forms it: the z width |nu_k+1 - nu_k| |t|
+            // is split into |t| (here, with the transverse width) and a per-row factor (in y')
+            const float wnorm = oa * __builtin_amdgcn_rcpf(wa * fabsf(t));
+            const float tl = fabsf(t);
+            // rows whose edges bracket the chunk [iz0, iz0 + nzc]: nu_e = (c - gam) / t,
+            // e = (nu_e - nuA) / nuB
+            const float r1 = ((fz0 - gam) * rt - A[11]) * A[12];
+            const float r2 = ((fz1 - gam) * rt - A[11]) * A[12];
+            const int k0 = max((int)floorf(fminf(r1, r2) - 0.01f), 0);
+            const int k1 = min((int)floorf(fmaxf(r1, r2) + 0.01f), nv - 1);
+            float csum[kAdjZChunk];
+#pragma unroll
+            for (int i = 0; i < kAdjZChunk; ++i) csum[i] = 0.0f;
+            for (int k = k0; k <= k1; ++k) {
+                const float c1 = fmaf(rows[k + 1], t, gam);
+                const float c2 = fmaf(rows[k], t, gam);
+                const float clo = fminf(c1, c2), chi = fmaxf(c1, c2);
+                const float thr = tol * fabsf(rows[k + 1] - rows[k]) * tl;
+                const float yk = yv[(size_t)k * nu + j];
+#pragma unroll
+                for (int i = 0; i < kAdjZChunk; ++i) {
+                    const float fkc = fz0 + (float)i;
                     const float oc = fminf(chi, fkc + 1.0f) - fmaxf(clo, fkc);
-                    if (!(oc > 0.0f) || oc < tol * wc) continue;
-                    const float s2 = rows[nv + 1 + k] * inv;
-                    const float L = sec * sqrtf(1.0f + s2 * s2);
-                    sum = fmaf(yv[(size_t)k * nu + j] * L, oc * __builtin_amdgcn_rcpf(wc), sum);
+                    const float w = (oc > 0.0f && oc >= thr) ? oc : 0.0f;
+                    csum[i] = fmaf(yk, w, csum[i]);
                 }
-                acc[i] = fmaf(sum, oa * __builtin_amdgcn_rcpf(wa), acc[i]);
             }
+#pragma unroll
+            for (int i = 0; i < kAdjZChunk; ++i) acc[i] = fmaf(csum[i], wnorm, acc[i]);
         }
     }
     for (int i = 0; i < nzc; ++i) {
@@ -1405,7 +1516,7 @@ int ctr_fdk_filter(const ctr_geometry* g, const double* spectrum, const float* p
 // Adjoint workspace: the forward projector's tables and row table (no transposed volume),
 // then the per-view adjoint constants.
 struct AdjLayout {
-    size_t tabs_off, rows_off, at_off, total;
+    size_t tabs_off, rows_off, at_off, yl_off, total;
 };
 
 AdjLayout adj_layout(const ctr_geometry* g) {
@@ -1413,7 +1524,8 @@ AdjLayout adj_layout(const ctr_geometry* g) {
     L.tabs_off = 0;
     L.rows_off = align_up((size_t)fp_view_stride(g->nu) * g->n_views * 4, 256);
     L.at_off = align_up(L.rows_off + (size_t)(2 * g->nv + 1) * 4, 256);
-    L.total = align_up(L.at_off + (size_t)g->n_views * kAdjViewStride * 4, 256);
+    L.yl_off = align_up(L.at_off + (size_t)g->n_views * kAdjViewStride * 4, 256);
+    L.total = align_up(L.yl_off + (size_t)g->n_views * g->nv * g->nu * 4, 256);
     return L;
 }
 
@@ -1441,6 +1553,7 @@ int ctr_dd_backproject(const ctr_geometry* g, const double* angles, const int32_
         float* tabs = (float*)(ws + L.tabs_off);
         float* rows = (float*)(ws + L.rows_off);
         float* at = (float*)(ws + L.at_off);
+        float* yl = (float*)(ws + L.yl_off);
         const int stride = fp_view_stride(g->nu);
         const int gx = (std::max(g->nu + 1, g->nv + 1) + 255) / 256;
         hipLaunchKernelGGL(fp_geometry_kernel, dim3(gx, g->n_views), dim3(256), 0, s, *g, angles,
@@ -1449,9 +1562,13 @@ int ctr_dd_backproject(const ctr_geometry* g, const double* angles, const int32_
         hipLaunchKernelGGL(adj_geometry_kernel, dim3((g->n_views + 255) / 256), dim3(256), 0, s, *g,
                            angles, branch, at);
         if ((st = status_from_hip(hipGetLastError()))) return st;
+        const int n_cells = g->n_views * g->nv * g->nu;
+        hipLaunchKernelGGL(adj_prescale_kernel, dim3((n_cells + 255) / 256), dim3(256), 0, s, proj, yl,
+                           tabs, rows, g->nu, g->nv, n_cells, stride);
+        if ((st = status_from_hip(hipGetLastError()))) return st;
         const dim3 block(64, 4);
         const dim3 grid((g->nx + 63) / 64, (g->ny + 3) / 4, (g->nz + kAdjZChunk - 1) / kAdjZChunk);
-        hipLaunchKernelGGL(dd_adjoint_kernel, grid, block, 0, s, proj, vol, tabs, rows, at, g->nx,
+        hipLaunchKernelGGL(dd_adjoint_kernel, grid, block, 0, s, yl, vol, tabs, rows, at, g->nx,
                            g->ny, g->nz, g->nu, g->nv, g->n_views, stride,
                            g->detector_shape == CTR_DETECTOR_CURVED ? 1 : 0, scale, 1e-6f, accumulate);
         return status_from_hip(hipGetLastError());

@@ -7,6 +7,9 @@
 //                         residual, stores bfloat16.  The input gradient of a convolution is
 //                         the same kernel applied to the output gradient with the weights
 //                         flipped in space and transposed in channels.
+//   conv3x3_fwd_lds_kernel the same for Cin in {32, 64}, Cout = 64, with the input tile and
+//                         its halo staged in LDS once per workgroup and the weights staged
+//                         one tap at a time.
 //   conv3x3_wgrad_kernel  weight and bias gradients: per pixel chunk, 64-pixel tiles of the
 //                         output gradient and of the shifted input staged in LDS, FP32 outer
 //                         products per (tap, 4 x 4 channel block); partial sums per chunk
@@ -49,6 +52,7 @@ __global__ void __launch_bounds__(256) conv3x3_fwd_kernel(
     const int m = lane & 15, g = lane >> 4;
     const int K = 9 * Cin;
     const int ksteps = (K + 31) / 32;
+    const int n_tiles = (Cout + 15) / 16;
     int py[2], px[2], pn[2];
     bool pv[2];
 #pragma unroll
@@ -115,7 +119,8 @@ __global__ void __launch_bounds__(256) conv3x3_fwd_kernel(
         for (int h = 0; h < 2; ++h)
 #pragma unroll
             for (int t = 0; t < 4; ++t)
-                acc[h][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[h], b[t], acc[h][t], 0, 0, 0);
+                if (t < n_tiles)  // uniform: column tiles past Cout are all zero
+                    acc[h][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[h], b[t], acc[h][t], 0, 0, 0);
     }
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
@@ -131,6 +136,126 @@ __global__ void __launch_bounds__(256) conv3x3_fwd_kernel(
                 float v = acc[h][t][r] + bc;
                 if (relu) v = fmaxf(v, 0.0f);
                 const size_t o = (size_t)p * Cout + co;
+                if (resid) v += (float)resid[o];
+                out[o] = (bf16_t)v;
+            }
+        }
+    }
+}
+
+// LDS-tiled variant for Cin in {32, 64} and Cout == 64 (the 64-channel body of the network).
+// A workgroup owns an 8 x 32 pixel tile of one image and all 64 output channels; the
+// (8+2) x (32+2) input halo is staged once in LDS (pixel stride Cin + 8 bf16, so that the
+// 16-byte operand reads of consecutive pixels fall on distinct banks), and the weights of one
+// tap at a time ([Cout][Cin], same padding) in a double buffer: the next tap's weights are
+// loaded into registers while the current tap runs.  Wave w computes rows 2w, 2w+1 of the
+// tile: 4 MFMA row tiles (16 pixels each) x 4 column tiles (16 channels), i.e. 16 MFMAs per
+// 8 LDS operand reads in each K step of 32.
+constexpr int kLdsTileH = 8, kLdsTileW = 32;
+constexpr int kHaloH = kLdsTileH + 2, kHaloW = kLdsTileW + 2;
+
+template <int CIN>
+__global__ void __launch_bounds__(256) conv3x3_fwd_lds_kernel(
+    const bf16_t* __restrict__ in, const bf16_t* __restrict__ w, const float* __restrict__ bias,
+    const bf16_t* __restrict__ resid, bf16_t* __restrict__ out, int H, int W, int relu) {
+    constexpr int CP = CIN + 8;          // padded pixel / weight-row stride in LDS (bf16)
+    constexpr int CHUNKS = CIN / 8;      // 16-byte chunks per pixel or weight row
+    constexpr int K = 9 * CIN;
+    constexpr int WCHUNKS = 64 * CHUNKS; // 16-byte chunks of one tap's weights
+    constexpr int WPER = (WCHUNKS + 255) / 256;
+    __shared__ __attribute__((aligned(16))) bf16_t halo[kHaloH * kHaloW * CP];
+    __shared__ __attribute__((aligned(16))) bf16_t wtap[2][64 * CP];
+    const int n = blockIdx.z;
+    const int ty0 = blockIdx.y * kLdsTileH, tx0 = blockIdx.x * kLdsTileW;
+    const int tid = threadIdx.x;
+    // stage the halo: (row, col, chunk) -> 16 bytes, zero outside the image
+    for (int e = tid; e < kHaloH * kHaloW * CHUNKS; e += 256) {
+        const int ch = e % CHUNKS;
+        const int pix = e / CHUNKS;
+        const int hx = pix % kHaloW, hy = pix / kHaloW;
+        const int yy = ty0 + hy - 1, xx = tx0 + hx - 1;
+        bf16x8 v = zero8();
+        if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+            v = *reinterpret_cast<const bf16x8*>(in + ((size_t)(n * H + yy) * W + xx) * CIN + ch * 8);
+        *reinterpret_cast<bf16x8*>(&halo[pix * CP + ch * 8]) = v;
+    }
+    // weights of tap 0
+#pragma unroll
+    for (int p = 0; p < WPER; ++p) {
+        const int e = tid + p * 256;
+        if (e < WCHUNKS) {
+            const int co = e / CHUNKS, ch = e % CHUNKS;
+            *reinterpret_cast<bf16x8*>(&wtap[0][co * CP + ch * 8]) =
+                *reinterpret_cast<const bf16x8*>(w + (size_t)co * K + ch * 8);
+        }
+    }
+    __syncthreads();
+    const int lane = tid & 63, wave = tid >> 6;
+    const int m = lane & 15, g = lane >> 4;
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[i][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    // M tile i: tile row 2*wave + (i >> 1), columns 16*(i & 1) .. +15
+#pragma unroll 1
+    for (int tap = 0; tap < 9; ++tap) {
+        const int buf = tap & 1;
+        bf16x8 wn[WPER];
+        if (tap + 1 < 9) {
+#pragma unroll
+            for (int p = 0; p < WPER; ++p) {
+                const int e = tid + p * 256;
+                const int co = e / CHUNKS, ch = e % CHUNKS;
+                if (e < WCHUNKS)
+                    wn[p] = *reinterpret_cast<const bf16x8*>(w + (size_t)co * K + (tap + 1) * CIN + ch * 8);
+            }
+        }
+        const int dy = tap / 3, dx = tap % 3;  // halo offsets (0..2)
+        const bf16_t* wt = wtap[buf];
+#pragma unroll
+        for (int c0 = 0; c0 < CIN; c0 += 32) {
+            bf16x8 b[4], a[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                b[t] = *reinterpret_cast<const bf16x8*>(&wt[(t * 16 + m) * CP + c0 + g * 8]);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int hy = 2 * wave + (i >> 1) + dy;
+                const int hx = 16 * (i & 1) + m + dx;
+                a[i] = *reinterpret_cast<const bf16x8*>(&halo[(hy * kHaloW + hx) * CP + c0 + g * 8]);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[t], acc[i][t], 0, 0, 0);
+        }
+        if (tap + 1 < 9) {
+#pragma unroll
+            for (int p = 0; p < WPER; ++p) {
+                const int e = tid + p * 256;
+                const int co = e / CHUNKS, ch = e % CHUNKS;
+                if (e < WCHUNKS) *reinterpret_cast<bf16x8*>(&wtap[buf ^ 1][co * CP + ch * 8]) = wn[p];
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int co = t * 16 + m;
+        const float bc = bias ? bias[co] : 0.0f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int yy = ty0 + 2 * wave + (i >> 1);
+            if (yy >= H) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int xx = tx0 + 16 * (i & 1) + 4 * g + r;
+                if (xx >= W) continue;
+                float v = acc[i][t][r] + bc;
+                if (relu) v = fmaxf(v, 0.0f);
+                const size_t o = ((size_t)(n * H + yy) * W + xx) * 64 + co;
                 if (resid) v += (float)resid[o];
                 out[o] = (bf16_t)v;
             }
@@ -251,7 +376,16 @@ int ctr_conv3x3_bf16_forward(const void* in, const void* weight, const float* bi
     const bf16_t* wt = (const bf16_t*)weight;
     const bf16_t* r = (const bf16_t*)residual;
     bf16_t* y = (bf16_t*)out;
-    if (cin % 32 == 0) {
+    if (cout == 64 && (cin == 64 || cin == 32) && n <= 65535) {
+        const dim3 grid((unsigned)((w + kLdsTileW - 1) / kLdsTileW), (unsigned)((h + kLdsTileH - 1) / kLdsTileH),
+                        (unsigned)n);
+        if (cin == 64)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(conv3x3_fwd_lds_kernel<64>), grid, dim3(256), 0, s, x, wt, bias,
+                               r, y, h, w, relu);
+        else
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(conv3x3_fwd_lds_kernel<32>), grid, dim3(256), 0, s, x, wt, bias,
+                               r, y, h, w, relu);
+    } else if (cin % 32 == 0) {
         hipLaunchKernelGGL(HIP_KERNEL_NAME(conv3x3_fwd_kernel<true>), dim3((unsigned)blocks), dim3(256),
                            0, s, x, wt, bias, r, y, n, h, w, cin, cout, relu);
     } else {
