@@ -28,7 +28,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-BP_KERNEL = "dd_backproject_tile_kernel<8>"  # the dominant kernel of one FDK step
+BP_KERNEL = "dd_backproject_tile_kernel<16>"  # the dominant kernel of one FDK step
 
 CONFIGS = {
     "c3": dict(n=512, voxel=0.25, views=720, nu=1024, nv=768, pitch=0.4, ellipsoids=40, seed=3,

@@ -1,14 +1,17 @@
 // ctrecon.hip -- gfx950 kernels and C ABI for the distance-driven CT projector pair and FDK.
 //
 // Kernels (all FP32 arithmetic; FP64 only in the per-view geometry kernels):
-//   fp_geometry_kernel     per-view DD footprint tables, FP64 -> FP32 (index space)
-//   transpose_xy_kernel    vol[z][y][x] -> vol_t[z][x][y] for x-slab views (coalesced gathers)
-//   dd_forward_kernel      one work-item per (view, detector cell); slabs looped in registers
-//   bp_geometry_kernel     per-view backprojection constants, FP64 -> FP32
-//   dd_backproject_kernel  one work-item per voxel column chunk; all views looped in registers
-//   spectrum_kernel        filter magnitude from the Ram-Lak cosine series, FP64
-//   twiddle_kernel         exp(-2 pi i k / N) table, FP64 -> FP32
-//   fdk_filter_kernel      cosine weight + LDS Stockham FFT convolution, two rows per FFT
+//   fp_geometry_kernel          per-view DD footprint tables, FP64 -> FP32 (index space)
+//   zfast_transpose_kernel      vol[z][y][x] -> z-fastest copies [y][x][z] and [x][y][z]
+//   dd_forward_column_kernel    one wavefront per (view, detector column, 64 rows)
+//   bp_geometry_kernel          per-view backprojection constants, FP64 -> FP32
+//   dd_backproject_tile_kernel  one workgroup per 16 x 16 x ZC voxel tile, windows in LDS
+//   adj_geometry_kernel         per-view constants of the inverse central projection
+//   adj_prescale_kernel         projections times path length over row width
+//   dd_adjoint_kernel           exact transpose of the forward projector, gathered per voxel
+//   spectrum_kernel             filter magnitude from the Ram-Lak cosine series, FP64
+//   twiddle_kernel              exp(-2 pi i k / N) table, FP64 -> FP32
+//   fdk_filter_kernel           cosine weight + LDS Stockham FFT convolution, two rows per FFT
 //
 // Reference semantics are cited at each kernel (leehoy/CTReconstruction paths).
 
@@ -70,17 +73,22 @@ int status_from_hip(hipError_t e) { return e == hipSuccess ? CTR_OK : CTR_ERR_HI
 inline int fp_view_stride(int nu) { return (int)align_up((size_t)(6 * nu + 3), 64); }
 
 struct FpLayout {
-    size_t tabs_off, rows_off, volt_off, total;
+    size_t tabs_off, rows_off, volt_off, copy_bytes, total;
 };
+
+// z stride of the z-fastest volume copies: two zero cells after every line, so that the three
+// z taps of a footprint can always be read as one 12-byte load
+int zfast_stride(int nz) { return nz + 2; }
 
 // workspace: tables, row table, then the volume twice with z fastest: [y][x][z] (gathered by
 // y-slab views) and [x][y][z] (x-slab views)
 FpLayout fp_layout(const ctr_geometry* g) {
     FpLayout L;
-    const size_t vbytes = align_up((size_t)g->nx * g->ny * g->nz * 4, 256);
+    const size_t vbytes = align_up((size_t)g->nx * g->ny * zfast_stride(g->nz) * 4, 256);
     L.tabs_off = 0;
     L.rows_off = align_up(L.tabs_off + (size_t)fp_view_stride(g->nu) * g->n_views * 4, 256);
     L.volt_off = align_up(L.rows_off + (size_t)(2 * g->nv + 1) * 4, 256);
+    L.copy_bytes = vbytes;
     L.total = L.volt_off + 2 * vbytes;
     return L;
 }
@@ -169,23 +177,6 @@ __global__ void fp_geometry_kernel(ctr_geometry g, const double* __restrict__ an
     if (e == 0) T[6 * nu + 2] = (float)((Sz - g.recon_center[2]) / (-g.dz) + g.nz / 2.0);
 }
 
-// vol[z][y][x] -> vol_t[z][x][y]; 32x32 tiles through LDS (padded against bank conflicts).
-__global__ void __launch_bounds__(256) transpose_xy_kernel(const float* __restrict__ in,
-                                                           float* __restrict__ out, int nx, int ny) {
-    __shared__ float tile[32][33];
-    const size_t zoff = (size_t)blockIdx.z * nx * ny;
-    const int x0 = blockIdx.x * 32, y0 = blockIdx.y * 32;
-    for (int r = threadIdx.y; r < 32; r += 8) {
-        const int y = y0 + r, x = x0 + threadIdx.x;
-        if (x < nx && y < ny) tile[r][threadIdx.x] = in[zoff + (size_t)y * nx + x];
-    }
-    __syncthreads();
-    for (int r = threadIdx.y; r < 32; r += 8) {
-        const int x = x0 + r, y = y0 + threadIdx.x;
-        if (x < nx && y < ny) out[zoff + (size_t)x * ny + y] = tile[threadIdx.x][r];
-    }
-}
-
 // Overlap weight (index space) of cell k with the footprint [lo, hi]; the cases
 // (single cell -> 1, first, last, interior) are those of python/GPUFuncs_pycuda.py:427-452.
 __device__ inline float dd_weight(int k, int k0, int k1, float lo, float hi, float inv_w,
@@ -199,157 +190,38 @@ __device__ inline float dd_weight(int k, int k0, int k1, float lo, float hi, flo
     return (w < zero_tol) ? 0.0f : w;
 }
 
-// DD forward projection, one work-item per (view, detector cell).  Kernel semantics of
-// distance_project_on_{y,x}3 (python/GPUFuncs_pycuda.py:398-524): for each slab the cell's
-// four boundary lines give a footprint [a1,a2] x [c1,c2] that is overlapped with the voxel
-// grid; here the reference's per-(slab, cell) atomicAdd is replaced by a register sum over
-// slabs and one coalesced store.  Coordinates are in index space (plane k at k), so the
-// same code serves increasing (x) and decreasing (y, z) planes.
-__global__ void __launch_bounds__(256) dd_forward_kernel(
-    const float* __restrict__ vol, const float* __restrict__ vol_t, float* __restrict__ proj,
-    const float* __restrict__ tabs, const float* __restrict__ rows,
-    const int32_t* __restrict__ branch, int nx, int ny, int nz, int nu, int nv, int stride,
-    float zero_tol, int n_views, int tiles_u, int n_blocks) {
-    // Workgroup b runs on XCD b % 8.  Each XCD gets a contiguous range of (detector tile, view)
-    // pairs, views fastest: concurrent workgroups then cast rays from the same detector tile at
-    // neighbouring angles, which cross the same thin z-slab of the volume (L2/MALL resident).
-    const int b = blockIdx.x;
-    const int xcd = b & 7, local = b >> 3, qd = n_blocks >> 3, rm = n_blocks & 7;
-    const int t = (xcd < rm) ? xcd * (qd + 1) + local : rm * (qd + 1) + (xcd - rm) * qd + local;
-    const int view = t % n_views;
-    const int tile = t / n_views;
-    const int j = (tile % tiles_u) * blockDim.x + threadIdx.x;
-    const int k = (tile / tiles_u) * blockDim.y + threadIdx.y;
-    if (j >= nu || k >= nv) return;
-    const int br = branch[view];
-    const size_t out_idx = ((size_t)view * nv + k) * nu + j;
-    if (br != CTR_BRANCH_X && br != CTR_BRANCH_Y) {
-        proj[out_idx] = 0.0f;
-        return;
-    }
-    const bool xs = (br == CTR_BRANCH_X);
-    const float* __restrict__ src = xs ? vol_t : vol;
-    const int ns = xs ? nx : ny;
-    const int nt = xs ? ny : nx;
-    const int s_stride = xs ? ny : nx;
-    const int v_stride = nx * ny;
-    const float* T = tabs + (size_t)view * stride;
-    const float a_al = T[j], b_al = T[j + 1];
-    const float a_be = T[nu + 1 + j], b_be = T[nu + 2 + j];
-    const float T0 = T[2 * nu + 2 + j], dT = T[3 * nu + 2 + j];
-    const float sec = T[4 * nu + 2 + j], inv = T[5 * nu + 2 + j];
-    const float gam = T[6 * nu + 2];
-    // V1 uses edge row k+1, V2 edge row k (python/Reconstruction_pycuda.py:844-846)
-    const float nu1 = rows[k + 1], nu2 = rows[k];
-    const float vc = rows[nv + 1 + k];
-    const float s2 = vc * inv;
-    const float Lk = sec * sqrtf(1.0f + s2 * s2);
-
-    float acc = 0.0f;
-    const float tol = zero_tol;
-    for (int n = 0; n < ns; ++n) {
-        const float fn = (float)n;
-        const float a1 = fmaf(a_be, fn, a_al);
-        const float a2 = fmaf(b_be, fn, b_al);
-        const float t = fmaf(dT, fn, T0);
-        const float c1 = fmaf(nu1, t, gam);
-        const float c2 = fmaf(nu2, t, gam);
-        const float alo = fminf(a1, a2), ahi = fmaxf(a1, a2);
-        const float clo = fminf(c1, c2), chi = fmaxf(c1, c2);
-        const int ka0 = (int)floorf(alo), ka1 = (int)floorf(ahi);
-        const int kc0 = (int)floorf(clo), kc1 = (int)floorf(chi);
-        const int kas = max(ka0, 0), kae = min(ka1, nt - 1);
-        const int kcs = max(kc0, 0), kce = min(kc1, nz - 1);
-        if (kas > kae || kcs > kce) continue;
-        // unnormalised overlaps o with the footprint widths: w = o / width (python/GPUFuncs_pycuda.py:427-452)
-        const float wa = ahi - alo, wc = chi - clo;
-        const float* slab = src + (size_t)n * s_stride;
-        float sum = 0.0f;
-        if (kae - kas <= 2 && kce - kcs <= 2) {
-            // footprint within 3 x 3 cells: fixed taps, all loads in flight.  Overlaps use the
-            // footprint clipped to the volume, so taps past its edge get o <= 0; addresses are
-            // clamped into the volume and use 32-bit byte offsets from the uniform slab base.
-            float oa[3], oc[3];
-            {
-                const float lo = fmaxf(alo, 0.0f), hi = fminf(ahi, (float)nt);
-                const float s1 = floorf(lo) + 1.0f;
-                oa[0] = fminf(hi, s1) - lo;
-                oa[1] = fminf(hi, s1 + 1.0f) - s1;
-                oa[2] = hi - (s1 + 1.0f);
-                const float tl = tol * wa;
-#pragma unroll
-                for (int i = 0; i < 3; ++i) oa[i] = (oa[i] >= tl) ? oa[i] : 0.0f;
-            }
-            {
-                const float lo = fmaxf(clo, 0.0f), hi = fminf(chi, (float)nz);
-                const float s1 = floorf(lo) + 1.0f;
-                oc[0] = fminf(hi, s1) - lo;
-                oc[1] = fminf(hi, s1 + 1.0f) - s1;
-                oc[2] = hi - (s1 + 1.0f);
-                const float tl = tol * wc;
-#pragma unroll
-                for (int i = 0; i < 3; ++i) oc[i] = (oc[i] >= tl) ? oc[i] : 0.0f;
-            }
-            const unsigned da = (unsigned)(kae - kas), dc = (unsigned)(kce - kcs);
-            const unsigned vs4 = 4u * (unsigned)v_stride;
-            const unsigned base = 4u * ((unsigned)kcs * (unsigned)v_stride + (unsigned)kas);
-            const unsigned ua[3] = {0u, 4u * min(1u, da), 4u * min(2u, da)};
-            const unsigned uc[3] = {0u, vs4 * min(1u, dc), vs4 * min(2u, dc)};
-            const char* sb = reinterpret_cast<const char*>(slab);
-            float f[3][3];
-#pragma unroll
-            for (int jz = 0; jz < 3; ++jz)
-#pragma unroll
-                for (int i = 0; i < 3; ++i)
-                    f[jz][i] = *reinterpret_cast<const float*>(sb + (base + uc[jz] + ua[i]));
-#pragma unroll
-            for (int jz = 0; jz < 3; ++jz) {
-                const float row = fmaf(oa[0], f[jz][0], fmaf(oa[1], f[jz][1], oa[2] * f[jz][2]));
-                sum = fmaf(oc[jz], row, sum);
-            }
-        } else {
-            for (int kc = kcs; kc <= kce; ++kc) {
-                const float p = fminf(chi, (float)(kc + 1)) - fmaxf(clo, (float)kc);
-                if (p < tol * wc) continue;
-                const float* line = slab + (size_t)kc * v_stride;
-                float row = 0.0f;
-                for (int ka = kas; ka <= kae; ++ka) {
-                    const float o = fminf(ahi, (float)(ka + 1)) - fmaxf(alo, (float)ka);
-                    if (o >= tol * wa) row = fmaf(line[ka], o, row);
-                }
-                sum = fmaf(row, p, sum);
-            }
-        }
-        // a single-cell footprint has o == width, i.e. weight 1 as the reference
-        acc = fmaf(sum, __builtin_amdgcn_rcpf(wa * wc), acc);
-    }
-    proj[out_idx] = acc * Lk;
-}
-
-// vol[z][y][x] -> v_ys[y][x][z] and v_xs[x][y][z]: 32 (x) x 32 (z) tiles of one y-plane
-// through LDS, coalesced along x on read and along z on write.
+// vol[z][y][x] -> v_ys[y][x][z] and v_xs[x][y][z] with z stride nzp (cells nz .. nzp-1 zero):
+// 32 (x) x 32 (z) tiles of one y-plane through LDS, coalesced along x on read and along z on
+// write.
 __global__ void __launch_bounds__(256) zfast_transpose_kernel(const float* __restrict__ in,
                                                               float* __restrict__ v_ys,
                                                               float* __restrict__ v_xs, int nx,
-                                                              int ny, int nz) {
+                                                              int ny, int nz, int nzp) {
     __shared__ float tile[32][33];
     const int x0 = blockIdx.x * 32, z0 = blockIdx.y * 32, y = blockIdx.z;
     for (int r = threadIdx.y; r < 32; r += 8) {
         const int z = z0 + r, x = x0 + threadIdx.x;
-        if (z < nz && x < nx) tile[r][threadIdx.x] = in[((size_t)z * ny + y) * nx + x];
+        tile[r][threadIdx.x] = (z < nz && x < nx) ? in[((size_t)z * ny + y) * nx + x] : 0.0f;
     }
     __syncthreads();
     for (int r = threadIdx.y; r < 32; r += 8) {
         const int x = x0 + r, z = z0 + threadIdx.x;
-        if (x < nx && z < nz) {
+        if (x < nx && z < nzp) {
             const float v = tile[threadIdx.x][r];
-            v_ys[((size_t)y * nx + x) * nz + z] = v;
-            v_xs[((size_t)x * ny + y) * nz + z] = v;
+            v_ys[((size_t)y * nx + x) * nzp + z] = v;
+            v_xs[((size_t)x * ny + y) * nzp + z] = v;
         }
     }
 }
 
-// DD forward projection with one wavefront per (view, detector column, 64 rows).  All rays of a
+// DD forward projection.  Kernel semantics of distance_project_on_{y,x}3
+// (python/GPUFuncs_pycuda.py:398-524): at every slab the cell's four boundary lines give a
+// footprint [a1,a2] x [c1,c2] that is overlapped with the voxel grid; the reference's
+// per-(slab, cell) atomicAdd becomes a register sum over slabs and one store.  Coordinates
+// are in index space (plane k at k), so the same code serves increasing (x) and decreasing
+// (y, z) planes.
+//
+// One wavefront per (view, detector column, 64 rows).  All rays of a
 // column share their transverse footprint at every slab (the U-edge lines depend on the column
 // only), so the transverse overlaps, the slab test and the transverse addresses are uniform
 // across the wavefront: lane L works them out for slab n0 + L of a batch of 64 slabs, and the
@@ -357,10 +229,21 @@ __global__ void __launch_bounds__(256) zfast_transpose_kernel(const float* __res
 // row's z footprint, c = nu * tt(n) + gamma for the row's two edge slopes nu, whose width
 // |nu1 - nu2| |tt(n)| factors into a per-lane constant and a per-slab scalar, so no per-lane
 // reciprocal is taken per slab.  The volume is read from a z-fastest copy, so the lanes' taps
-// are consecutive in memory.  Same semantics and weights as dd_forward_kernel
-// (python/GPUFuncs_pycuda.py:398-524): overlaps below zero_tol times the footprint width are
-// dropped, a single-cell footprint has weight 1.
+// are consecutive in memory.  Overlaps below zero_tol times the footprint width are dropped and
+// a single-cell footprint has weight 1, as in the reference.
 constexpr int kColWaves = 4;
+// Views per block of the workgroup order: consecutive workgroups on one XCD take kViewBlock
+// adjacent views of one column group, then the next column group, so the workgroups resident
+// on an XCD at one time trace nearby rays and share volume lines in its L2.
+#ifndef CTR_FP_VIEW_BLOCK
+#define CTR_FP_VIEW_BLOCK 32
+#endif
+constexpr int kViewBlock = CTR_FP_VIEW_BLOCK;
+
+// three consecutive z cells of a z-fastest line (4-byte aligned)
+struct __attribute__((packed, aligned(4))) ZTaps {
+    float a, b, c;
+};
 
 __device__ inline float lane_f(float v, int q) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), q));
@@ -369,14 +252,18 @@ __device__ inline float lane_f(float v, int q) {
 __global__ void __launch_bounds__(256) dd_forward_column_kernel(
     const float* __restrict__ v_ys, const float* __restrict__ v_xs, float* __restrict__ proj,
     const float* __restrict__ tabs, const float* __restrict__ rows,
-    const int32_t* __restrict__ branch, int nx, int ny, int nz, int nu, int nv, int stride,
-    float zero_tol, int n_views, int col_groups, int n_blocks) {
+    const int32_t* __restrict__ branch, int nx, int ny, int nz, int nzp, int nu, int nv, int stride,
+    float zero_tol, int n_views, int col_groups, int n_vblocks, int n_blocks) {
     const int b = blockIdx.x;
     const int xcd = b & 7, local = b >> 3, qd = n_blocks >> 3, rm = n_blocks & 7;
     const int t = (xcd < rm) ? xcd * (qd + 1) + local : rm * (qd + 1) + (xcd - rm) * qd + local;
-    const int view = t % n_views;
-    const int rest = t / n_views;
-    const int cg = rest % col_groups, rb = rest / col_groups;
+    const int vi = t % kViewBlock;
+    const int r1 = t / kViewBlock;
+    const int cg = r1 % col_groups;
+    const int r2 = r1 / col_groups;
+    const int view = (r2 % n_vblocks) * kViewBlock + vi;
+    const int rb = r2 / n_vblocks;
+    if (view >= n_views) return;
     const int lane = threadIdx.x & 63;
     // wave-uniform: lets the compiler keep column data and addresses in scalar registers
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
@@ -395,7 +282,7 @@ __global__ void __launch_bounds__(256) dd_forward_column_kernel(
     const float* __restrict__ src = xs ? v_xs : v_ys;
     const int ns = xs ? nx : ny;
     const int nt = xs ? ny : nx;
-    const size_t slab_stride = (size_t)nt * nz;
+    const size_t slab_stride = (size_t)nt * nzp;
     const float* T = tabs + (size_t)view * stride;
     const float a_al = T[j], b_al = T[j + 1];
     const float a_be = T[nu + 1 + j], b_be = T[nu + 2 + j];
@@ -409,12 +296,39 @@ __global__ void __launch_bounds__(256) dd_forward_column_kernel(
     const float Lk = sec * sqrtf(1.0f + s2 * s2) / dnu;
     const float tol_c = zero_tol * dnu;
     const float fnz = (float)nz, fnt = (float)nt;
+    // z extent of the whole wavefront: its rows' edge slopes run from nu_first to nu_last, and
+    // no row's footprint is wider than dnu_max |tt|
+    const int e_first = rb * 64, e_last = min(rb * 64 + 64, nv);
+    const float nu_first = rows[e_first], nu_last = rows[e_last];
+    float dnu_max = dnu;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) dnu_max = fmaxf(dnu_max, __shfl_xor(dnu_max, o));
+    dnu_max = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(dnu_max)));
     float acc = 0.0f;
+    // z overlaps of this lane's footprint with cells kcs .. kcs+2 and the byte offset of cell
+    // kcs (taps past the footprint read the next cells, or the zero padding, with weight 0; a
+    // footprint outside the volume reads from cell nz - 1 with all weights 0)
+    auto z_taps = [&](float tt, float& o0, float& o1, float& o2, unsigned& v0) {
+        const float c1 = fmaf(nu1, tt, gam), c2 = fmaf(nu2, tt, gam);
+        const float clo = fminf(c1, c2), chi = fmaxf(c1, c2);
+        const int kcs = max((int)floorf(clo), 0);
+        const float tl = tol_c * fabsf(tt);  // zero_tol * wc
+        const float lo = fmaxf(clo, 0.0f), hi = fminf(chi, fnz);
+        const float s1 = (float)kcs + 1.0f;
+        o0 = fminf(hi, s1) - lo;
+        o1 = fminf(hi, s1 + 1.0f) - s1;
+        o2 = hi - (s1 + 1.0f);
+        o0 = (o0 >= tl) ? o0 : 0.0f;
+        o1 = (o1 >= tl) ? o1 : 0.0f;
+        o2 = (o2 >= tl) ? o2 : 0.0f;
+        v0 = 4u * (unsigned)min(kcs, nz - 1);
+    };
     for (int n0 = 0; n0 < ns; n0 += 64) {
-        // lane L: transverse footprint of slab n0 + L.  Mode 0: misses the volume, 1: at most
-        // three cells (weights pre-divided by wa |tt|), 2: wider, explicit loop.
+        // lane L: transverse footprint of slab n0 + L.  Mode 0: misses the volume (transverse,
+        // or in z for every row of the wavefront), 1: at most three cells in both directions
+        // (weights pre-divided by wa |tt|), 2: wider, explicit loop.
         float b_tt = 1.0f, b_w0 = 0.0f, b_w1 = 0.0f, b_w2 = 0.0f, b_alo = 0.0f, b_ahi = 0.0f;
-        int b_pack = 0;
+        int b_kas = 0, b_da = 0, b_mode = 0;
         {
             const int n = n0 + lane;
             if (n < ns) {
@@ -422,10 +336,13 @@ __global__ void __launch_bounds__(256) dd_forward_column_kernel(
                 const float a1 = fmaf(a_be, fn, a_al), a2 = fmaf(b_be, fn, b_al);
                 const float alo = fminf(a1, a2), ahi = fmaxf(a1, a2);
                 const int kas = max((int)floorf(alo), 0), kae = min((int)floorf(ahi), nt - 1);
-                b_tt = fmaf(dT, fn, T0);
+                const float tt = fmaf(dT, fn, T0);
+                const float za = fmaf(nu_first, tt, gam), zb = fmaf(nu_last, tt, gam);
+                const bool z_hit = fmaxf(za, zb) >= -0.01f && fminf(za, zb) <= fnz + 0.01f;
+                b_tt = tt;
                 b_alo = alo;
                 b_ahi = ahi;
-                if (kas <= kae) {
+                if (kas <= kae && z_hit) {
                     const int da = kae - kas;
                     const float wa = ahi - alo;
                     // overlaps with cells kas .. kas+2 (floor(max(alo, 0)) == kas); negative
@@ -437,19 +354,79 @@ __global__ void __launch_bounds__(256) dd_forward_column_kernel(
                     o0 = (o0 >= tl) ? o0 : 0.0f;
                     o1 = (o1 >= tl) ? o1 : 0.0f;
                     o2 = (o2 >= tl) ? o2 : 0.0f;
-                    const float r = __builtin_amdgcn_rcpf(wa * fabsf(b_tt));
+                    const float r = __builtin_amdgcn_rcpf(wa * fabsf(tt));
                     b_w0 = o0 * r;
                     b_w1 = o1 * r;
                     b_w2 = o2 * r;
-                    b_pack = (da <= 2) ? ((1 << 28) | (da << 24) | kas) : (2 << 28);
+                    b_kas = kas;
+                    b_da = da;
+                    // z footprints no wider than 1.99 cells touch at most three cells
+                    b_mode = (da <= 2 && dnu_max * fabsf(tt) <= 1.99f) ? 1 : 2;
                 }
             }
         }
-        const int nb = min(64, ns - n0);
-        for (int q = 0; q < nb; ++q) {
-            const int pack = __builtin_amdgcn_readlane(b_pack, q);
-            const int mode = (pack >> 28) & 3;
-            if (mode == 0) continue;
+        // mode-1 slabs, software-pipelined: the taps of the next slab are loaded while the
+        // current one is summed
+        uint64_t fast = __ballot(b_mode == 1);
+        if (fast) {
+            float f[9], o[3], w[3];
+            auto issue = [&](int q, float (&ff)[9], float (&oo)[3], float (&ww)[3]) {
+                const float tt = lane_f(b_tt, q);
+                unsigned v0;
+                z_taps(tt, oo[0], oo[1], oo[2], v0);
+                const int kas = __builtin_amdgcn_readlane(b_kas, q);
+                const int da = __builtin_amdgcn_readlane(b_da, q);
+                const float* slab = src + (size_t)(n0 + q) * slab_stride;
+                const char* r0 = reinterpret_cast<const char*>(slab + (size_t)kas * nzp);
+                const char* r1 = reinterpret_cast<const char*>(slab + (size_t)(kas + min(1, da)) * nzp);
+                const char* r2 = reinterpret_cast<const char*>(slab + (size_t)(kas + min(2, da)) * nzp);
+                const ZTaps t0 = *reinterpret_cast<const ZTaps*>(r0 + v0);
+                const ZTaps t1 = *reinterpret_cast<const ZTaps*>(r1 + v0);
+                const ZTaps t2 = *reinterpret_cast<const ZTaps*>(r2 + v0);
+                ff[0] = t0.a; ff[1] = t0.b; ff[2] = t0.c;
+                ff[3] = t1.a; ff[4] = t1.b; ff[5] = t1.c;
+                ff[6] = t2.a; ff[7] = t2.b; ff[8] = t2.c;
+                ww[0] = lane_f(b_w0, q);
+                ww[1] = lane_f(b_w1, q);
+                ww[2] = lane_f(b_w2, q);
+            };
+            auto consume = [&](const float (&ff)[9], const float (&oo)[3], const float (&ww)[3]) {
+                const float col0 = fmaf(oo[0], ff[0], fmaf(oo[1], ff[1], oo[2] * ff[2]));
+                const float col1 = fmaf(oo[0], ff[3], fmaf(oo[1], ff[4], oo[2] * ff[5]));
+                const float col2 = fmaf(oo[0], ff[6], fmaf(oo[1], ff[7], oo[2] * ff[8]));
+                acc = fmaf(ww[0], col0, fmaf(ww[1], col1, fmaf(ww[2], col2, acc)));
+            };
+            // two register sets used in turn, so no copy waits on the loads in flight
+            float g[9], p[3], x[3];
+            int q = __builtin_ctzll(fast);
+            fast &= fast - 1;
+            issue(q, f, o, w);
+            while (true) {
+                if (fast) {
+                    q = __builtin_ctzll(fast);
+                    fast &= fast - 1;
+                    issue(q, g, p, x);
+                    consume(f, o, w);
+                } else {
+                    consume(f, o, w);
+                    break;
+                }
+                if (fast) {
+                    q = __builtin_ctzll(fast);
+                    fast &= fast - 1;
+                    issue(q, f, o, w);
+                    consume(g, p, x);
+                } else {
+                    consume(g, p, x);
+                    break;
+                }
+            }
+        }
+        // mode-2 slabs: explicit loops over every cell of the footprint
+        uint64_t wide = __ballot(b_mode == 2);
+        while (wide) {
+            const int q = __builtin_ctzll(wide);
+            wide &= wide - 1;
             const float tt = lane_f(b_tt, q);
             const float att = fabsf(tt);
             const float* slab = src + (size_t)(n0 + q) * slab_stride;
@@ -457,56 +434,23 @@ __global__ void __launch_bounds__(256) dd_forward_column_kernel(
             const float clo = fminf(c1, c2), chi = fmaxf(c1, c2);
             const int kcs = max((int)floorf(clo), 0), kce = min((int)floorf(chi), nz - 1);
             const float tl = tol_c * att;  // zero_tol * wc
-            if (mode == 1 && !__any(kce - kcs > 2)) {
-                const int kas = pack & 0xffffff, da = (pack >> 24) & 15;
-                const float lo = fmaxf(clo, 0.0f), hi = fminf(chi, fnz);
-                const float s1 = (float)kcs + 1.0f;
-                float o0 = fminf(hi, s1) - lo, o1 = fminf(hi, s1 + 1.0f) - s1, o2 = hi - (s1 + 1.0f);
-                o0 = (o0 >= tl) ? o0 : 0.0f;
-                o1 = (o1 >= tl) ? o1 : 0.0f;
-                o2 = (o2 >= tl) ? o2 : 0.0f;
-                // taps past the footprint repeat its last cell (weight 0); a footprint outside
-                // the volume reads cell nz - 1 with all weights 0
-                const int zc = min(kcs, nz - 1);
-                const int dc = max(kce - zc, 0);
-                const unsigned v0 = 4u * (unsigned)zc;
-                const unsigned v1 = v0 + 4u * (unsigned)min(1, dc), v2 = v0 + 4u * (unsigned)min(2, dc);
-                const float w0 = lane_f(b_w0, q), w1 = lane_f(b_w1, q), w2 = lane_f(b_w2, q);
-                const char* r0 = reinterpret_cast<const char*>(slab + (size_t)kas * nz);
-                const char* r1 = reinterpret_cast<const char*>(slab + (size_t)(kas + min(1, da)) * nz);
-                const char* r2 = reinterpret_cast<const char*>(slab + (size_t)(kas + min(2, da)) * nz);
-                const float f00 = *reinterpret_cast<const float*>(r0 + v0);
-                const float f01 = *reinterpret_cast<const float*>(r0 + v1);
-                const float f02 = *reinterpret_cast<const float*>(r0 + v2);
-                const float f10 = *reinterpret_cast<const float*>(r1 + v0);
-                const float f11 = *reinterpret_cast<const float*>(r1 + v1);
-                const float f12 = *reinterpret_cast<const float*>(r1 + v2);
-                const float f20 = *reinterpret_cast<const float*>(r2 + v0);
-                const float f21 = *reinterpret_cast<const float*>(r2 + v1);
-                const float f22 = *reinterpret_cast<const float*>(r2 + v2);
-                const float col0 = fmaf(o0, f00, fmaf(o1, f01, o2 * f02));
-                const float col1 = fmaf(o0, f10, fmaf(o1, f11, o2 * f12));
-                const float col2 = fmaf(o0, f20, fmaf(o1, f21, o2 * f22));
-                acc = fmaf(w0, col0, fmaf(w1, col1, fmaf(w2, col2, acc)));
-            } else {
-                const float alo = lane_f(b_alo, q), ahi = lane_f(b_ahi, q);
-                const float wa = ahi - alo;
-                const int kas = max((int)floorf(alo), 0), kae = min((int)floorf(ahi), nt - 1);
-                if (kcs > kce) continue;
-                float sum = 0.0f;
-                for (int ka = kas; ka <= kae; ++ka) {
-                    const float o = fminf(ahi, (float)(ka + 1)) - fmaxf(alo, (float)ka);
-                    if (o < zero_tol * wa) continue;
-                    const float* line = slab + (size_t)ka * nz;
-                    float col = 0.0f;
-                    for (int kc = kcs; kc <= kce; ++kc) {
-                        const float p = fminf(chi, (float)(kc + 1)) - fmaxf(clo, (float)kc);
-                        if (p >= tl) col = fmaf(line[kc], p, col);
-                    }
-                    sum = fmaf(col, o, sum);
+            const float alo = lane_f(b_alo, q), ahi = lane_f(b_ahi, q);
+            const float wa = ahi - alo;
+            const int kas = max((int)floorf(alo), 0), kae = min((int)floorf(ahi), nt - 1);
+            if (kcs > kce) continue;
+            float sum = 0.0f;
+            for (int ka = kas; ka <= kae; ++ka) {
+                const float ov = fminf(ahi, (float)(ka + 1)) - fmaxf(alo, (float)ka);
+                if (ov < zero_tol * wa) continue;
+                const float* line = slab + (size_t)ka * nzp;
+                float col = 0.0f;
+                for (int kc = kcs; kc <= kce; ++kc) {
+                    const float p = fminf(chi, (float)(kc + 1)) - fmaxf(clo, (float)kc);
+                    if (p >= tl) col = fmaf(line[kc], p, col);
                 }
-                acc = fmaf(sum, __builtin_amdgcn_rcpf(wa * att), acc);
+                sum = fmaf(col, ov, sum);
             }
+            acc = fmaf(sum, __builtin_amdgcn_rcpf(wa * att), acc);
         }
     }
     if (active) proj[out_idx] = acc * Lk;
@@ -517,7 +461,6 @@ __global__ void __launch_bounds__(256) dd_forward_column_kernel(
 // ------------------------------------------------------------------------------------------
 
 constexpr int kBpViewStride = 16;  // floats per view in the backprojection table
-constexpr int kBpZChunk = 8;       // voxels along z per work-item
 
 // Per-view constants of flat_distance_backproj_arb2's param vector
 // (python/Reconstruction_pycuda.py:405-407, 421-423), in FP64, stored as FP32:
@@ -571,101 +514,8 @@ __global__ void bp_geometry_kernel(ctr_geometry g, const double* __restrict__ an
 // With angle2 = 0 the u-footprint depends only on the voxel column, so it is computed once
 // per (column, view) and reused along z; the reference's per-tap atomicAdd becomes a
 // register accumulation over all views and one store.
-__global__ void __launch_bounds__(256) dd_backproject_kernel(
-    const float* __restrict__ q, float* __restrict__ vol, const float* __restrict__ vt,
-    int nx, int ny, int nz, int nu, int nv, int n_views, float x0, float y0, float z0,
-    float dx, float dy, float dz, int curved, float scale, float zero_tol, int accumulate) {
-    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
-    const int iy = blockIdx.y * blockDim.y + threadIdx.y;
-    const int iz0 = blockIdx.z * kBpZChunk;
-    if (ix >= nx || iy >= ny) return;
-    // voxel centre (python/Reconstruction_pycuda.py:350-352); dy, dz are the signed steps
-    const float X = fmaf((float)ix, dx, x0);
-    const float Y = fmaf((float)iy, dy, y0);
-    const float hx = 0.5f * fabsf(dx), hy = 0.5f * fabsf(dy), hz = 0.5f * fabsf(dz);
-    float acc[kBpZChunk];
-#pragma unroll
-    for (int i = 0; i < kBpZChunk; ++i) acc[i] = 0.0f;
-
-    for (int view = 0; view < n_views; ++view) {
-        const float* T = vt + (size_t)view * kBpViewStride;
-        const float c = T[0], s = T[1], Sx = T[2], Sy = T[3], Sz = T[4];
-        const float Au = T[5], Bu = T[6], Av = T[7], Bv = T[8], R = T[9];
-        const float xr0 = X * c + Y * s;
-        const float yr0 = Y * c - X * s;
-        // four in-plane corners (the z pairs share them)
-        float smin = 3.0e38f, smax = -3.0e38f, rmin = 3.0e38f, rmax = -3.0e38f;
-#pragma unroll
-        for (int corner = 0; corner < 4; ++corner) {
-            const float sx = (corner & 1) ? -hx : hx;
-            const float sy = (corner & 2) ? -hy : hy;
-            const float xr = xr0 + sx * c + sy * s;
-            const float yr = yr0 + sy * c - sx * s;
-            const float r = 1.0f / (Sy - yr);
-            const float su = (Sx - xr) * r;
-            smin = fminf(smin, su);
-            smax = fmaxf(smax, su);
-            rmin = fminf(rmin, r);
-            rmax = fmaxf(rmax, r);
-        }
-        float fu1, fu2;
-        if (curved) {
-            fu1 = Au - Bu * atanf(smin);
-            fu2 = Au - Bu * atanf(smax);
-        } else {
-            fu1 = Au - Bu * smin;
-            fu2 = Au - Bu * smax;
-        }
-        const float ulo = fminf(fu1, fu2), uhi = fmaxf(fu1, fu2);
-        const int ku0 = (int)floorf(ulo), ku1 = (int)floorf(uhi);
-        const int kus = max(ku0, 0), kue = min(ku1, nu - 1);
-        if (kus > kue) continue;
-        const float inv_u = 1.0f / (uhi - ulo);
-        const float dd = R - yr0;
-        const float W = scale * (R * R) / (dd * dd);
-        const float* qv = q + (size_t)view * nv * nu;
-#pragma unroll
-        for (int i = 0; i < kBpZChunk; ++i) {
-            const int iz = iz0 + i;
-            if (iz >= nz) break;
-            const float Z = fmaf((float)iz, dz, z0);
-            const float e = Sz - Z;
-            // slope_v = (Sz - z') / (Sy - y') over the 8 corners: extremes at r_min / r_max
-            const float ep = e + hz, em = e - hz;
-            const float vmax = ep * (ep >= 0.0f ? rmax : rmin);
-            const float vmin = em * (em >= 0.0f ? rmin : rmax);
-            const float fv1 = Av - Bv * vmin;
-            const float fv2 = Av - Bv * vmax;
-            const float vlo = fminf(fv1, fv2), vhi = fmaxf(fv1, fv2);
-            const int kv0 = (int)floorf(vlo), kv1 = (int)floorf(vhi);
-            const int kvs = max(kv0, 0), kve = min(kv1, nv - 1);
-            if (kvs > kve) continue;
-            const float inv_v = 1.0f / (vhi - vlo);
-            float sum = 0.0f;
-            for (int kv = kvs; kv <= kve; ++kv) {
-                const float wv = dd_weight(kv, kv0, kv1, vlo, vhi, inv_v, zero_tol);
-                const float* row = qv + (size_t)kv * nu;
-                float rs = 0.0f;
-                for (int ku = kus; ku <= kue; ++ku) {
-                    const float wu = dd_weight(ku, ku0, ku1, ulo, uhi, inv_u, zero_tol);
-                    rs = fmaf(row[ku], wu, rs);
-                }
-                sum = fmaf(rs, wv, sum);
-            }
-            acc[i] = fmaf(sum, W, acc[i]);
-        }
-    }
-#pragma unroll
-    for (int i = 0; i < kBpZChunk; ++i) {
-        const int iz = iz0 + i;
-        if (iz >= nz) break;
-        const size_t idx = ((size_t)iz * ny + iy) * nx + ix;
-        vol[idx] = accumulate ? vol[idx] + acc[i] : acc[i];
-    }
-}
-
-// Tiled FDK backprojection: the same per-voxel semantics as dd_backproject_kernel, organised
-// for gfx950.  A workgroup owns a 16 x 16 x ZC voxel tile; for every view it stages the detector
+//
+// Organisation: a workgroup owns a 16 x 16 x ZC voxel tile; for every view it stages the detector
 // window that the tile's bounding box projects to in LDS (double-buffered, one barrier per
 // view).  Each work-item computes the u-overlap weights of its voxel column once per view,
 // filters every window row it needs in u once, and keeps the running sum of those rows (a row
@@ -680,7 +530,14 @@ constexpr int kTileX = 16, kTileY = 16;
 constexpr int kWinCap = 1024;          // floats per LDS window buffer
 constexpr int kMaxUTaps = 3;           // u taps held in registers (more are looped)
 constexpr int kPrefetch = kWinCap / 256;
-constexpr int kMaxRows = 16;           // rows per column chunk on the row-integral path
+#ifndef CTR_BP_ZC
+#define CTR_BP_ZC 16
+#endif
+#ifndef CTR_BP_MAX_ROWS
+#define CTR_BP_MAX_ROWS 24
+#endif
+constexpr int kBpZC = CTR_BP_ZC;            // voxels along z per work-item
+constexpr int kMaxRows = CTR_BP_MAX_ROWS;   // rows per column chunk on the row-integral path
 
 struct BpView {
     float c, s, Sx, Sy, Sz, Au, Bu, Av, Bv, R;
@@ -1003,7 +860,7 @@ __global__ void __launch_bounds__(256) dd_backproject_tile_kernel(
 // Matched adjoint of the forward projector
 // ------------------------------------------------------------------------------------------
 
-// The exact transpose of dd_forward_kernel, as a gather over voxels (no atomics).  No reference
+// The exact transpose of dd_forward_column_kernel, as a gather over voxels (no atomics).  No reference
 // counterpart: the reference backprojects with the FDK-weighted operator, which is not A^T.
 // For one view a voxel sits in slab n at transverse cell ka and z cell kc (index space of the
 // forward projector's tables).  The detector columns whose U-edge lines bracket [ka, ka+1] at
@@ -1442,22 +1299,24 @@ int ctr_dd_forward(const ctr_geometry* g, const double* angles, const int32_t* b
     float* tabs = (float*)(ws + L.tabs_off);
     float* rows = (float*)(ws + L.rows_off);
     float* v_ys = (float*)(ws + L.volt_off);
-    float* v_xs = v_ys + align_up((size_t)g->nx * g->ny * g->nz * 4, 256) / 4;
+    float* v_xs = (float*)(ws + L.volt_off + L.copy_bytes);
     const int stride = fp_view_stride(g->nu);
     const int gx = (std::max(g->nu + 1, g->nv + 1) + 255) / 256;
     hipLaunchKernelGGL(fp_geometry_kernel, dim3(gx, g->n_views), dim3(256), 0, s, *g, angles, branch,
                        tabs, rows, stride);
     if ((st = status_from_hip(hipGetLastError()))) return st;
-    hipLaunchKernelGGL(zfast_transpose_kernel, dim3((g->nx + 31) / 32, (g->nz + 31) / 32, g->ny),
-                       dim3(32, 8), 0, s, vol, v_ys, v_xs, g->nx, g->ny, g->nz);
+    const int nzp = zfast_stride(g->nz);
+    hipLaunchKernelGGL(zfast_transpose_kernel, dim3((g->nx + 31) / 32, (nzp + 31) / 32, g->ny),
+                       dim3(32, 8), 0, s, vol, v_ys, v_xs, g->nx, g->ny, g->nz, nzp);
     if ((st = status_from_hip(hipGetLastError()))) return st;
     const int col_groups = (g->nu + kColWaves - 1) / kColWaves;
     const int row_blocks = (g->nv + 63) / 64;
-    const long long n_blocks = (long long)col_groups * row_blocks * g->n_views;
+    const int n_vblocks = (g->n_views + kViewBlock - 1) / kViewBlock;
+    const long long n_blocks = (long long)col_groups * row_blocks * n_vblocks * kViewBlock;
     if (n_blocks >= 2147483647LL) return CTR_ERR_INVALID;
     hipLaunchKernelGGL(dd_forward_column_kernel, dim3((unsigned)n_blocks), dim3(256), 0, s, v_ys, v_xs,
-                       proj, tabs, rows, branch, g->nx, g->ny, g->nz, g->nu, g->nv, stride, 1e-6f,
-                       g->n_views, col_groups, (int)n_blocks);
+                       proj, tabs, rows, branch, g->nx, g->ny, g->nz, nzp, g->nu, g->nv, stride, 1e-6f,
+                       g->n_views, col_groups, n_vblocks, (int)n_blocks);
     return status_from_hip(hipGetLastError());
 }
 
@@ -1581,7 +1440,7 @@ int ctr_dd_backproject(const ctr_geometry* g, const double* angles, const int32_
     const float x0 = (float)(g->recon_center[0] - (g->nx - 1) / 2.0 * g->dx);
     const float y0 = (float)(g->recon_center[1] + (g->ny - 1) / 2.0 * g->dy);
     const float z0 = (float)(g->recon_center[2] + (g->nz - 1) / 2.0 * g->dz);
-    constexpr int ZC = 8;
+    constexpr int ZC = kBpZC;
     const int tiles_x = (g->nx + kTileX - 1) / kTileX;
     const int tiles_y = (g->ny + kTileY - 1) / kTileY;
     const int tiles_z = (g->nz + ZC - 1) / ZC;
