@@ -7,9 +7,9 @@
 //                         residual, stores bfloat16.  The input gradient of a convolution is
 //                         the same kernel applied to the output gradient with the weights
 //                         flipped in space and transposed in channels.
-//   conv3x3_fwd_lds_kernel the same for Cin in {32, 64}, Cout = 64, with the input tile and
-//                         its halo staged in LDS once per workgroup and the weights staged
-//                         one tap at a time.
+//   conv3x3_fwd_persistent_kernel  the same for 64 -> 64 channels: one workgroup per CU with
+//                         all weights in LDS, walking pixel tiles with the next tile's
+//                         input halo loaded while the current one is computed.
 //   conv3x3_wgrad_kernel  weight and bias gradients: per pixel chunk, 64-pixel tiles of the
 //                         output gradient and of the shifted input staged in LDS, FP32 outer
 //                         products per (tap, 4 x 4 channel block); partial sums per chunk
@@ -17,6 +17,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "ctrecon.h"
@@ -143,123 +144,171 @@ __global__ void __launch_bounds__(256) conv3x3_fwd_kernel(
     }
 }
 
-// LDS-tiled variant for Cin in {32, 64} and Cout == 64 (the 64-channel body of the network).
-// A workgroup owns an 8 x 32 pixel tile of one image and all 64 output channels; the
-// (8+2) x (32+2) input halo is staged once in LDS (pixel stride Cin + 8 bf16, so that the
-// 16-byte operand reads of consecutive pixels fall on distinct banks), and the weights of one
-// tap at a time ([Cout][Cin], same padding) in a double buffer: the next tap's weights are
-// loaded into registers while the current tap runs.  Wave w computes rows 2w, 2w+1 of the
-// tile: 4 MFMA row tiles (16 pixels each) x 4 column tiles (16 channels), i.e. 16 MFMAs per
-// 8 LDS operand reads in each K step of 32.
-constexpr int kLdsTileH = 8, kLdsTileW = 32;
-constexpr int kHaloH = kLdsTileH + 2, kHaloW = kLdsTileW + 2;
+// Persistent LDS variant for the 64 -> 64 channel body of the network.  One 512-thread
+// workgroup per CU keeps all nine taps of the weights in LDS (72 KB) and walks 16 x 32 pixel
+// tiles: the (16+2) x (32+2) input halo of the current tile sits in LDS (76.5 KB) while the
+// next tile's halo is already loaded into registers.  Wave w computes rows 2w, 2w+1 of the
+// tile: 4 MFMA row tiles (16 pixels) x 4 column tiles (16 channels), 16 MFMAs per 8 LDS
+// operand reads in each K step of 32.  Rows of 8 16-byte chunks are stored with the chunk
+// index XOR-ed by (row >> 1) & 7, so the operand reads of 16 consecutive pixels (or output
+// channels) hit 16 different bank groups without padding.
+constexpr int kPTileH = 16, kPTileW = 32;
+constexpr int kPHaloW = kPTileW + 2;
+constexpr int kPHaloPix = (kPTileH + 2) * kPHaloW;
+constexpr int kPHaloChunks = kPHaloPix * 8;
+constexpr int kPThreads = 512;
+constexpr int kPPer = (kPHaloChunks + kPThreads - 1) / kPThreads;
 
-template <int CIN>
-__global__ void __launch_bounds__(256) conv3x3_fwd_lds_kernel(
+__device__ inline int swz(int row, int chunk) { return chunk ^ ((row >> 1) & 7); }
+
+__global__ void __launch_bounds__(kPThreads) conv3x3_fwd_persistent_kernel(
     const bf16_t* __restrict__ in, const bf16_t* __restrict__ w, const float* __restrict__ bias,
-    const bf16_t* __restrict__ resid, bf16_t* __restrict__ out, int H, int W, int relu) {
-    constexpr int CP = CIN + 8;          // padded pixel / weight-row stride in LDS (bf16)
-    constexpr int CHUNKS = CIN / 8;      // 16-byte chunks per pixel or weight row
-    constexpr int K = 9 * CIN;
-    constexpr int WCHUNKS = 64 * CHUNKS; // 16-byte chunks of one tap's weights
-    constexpr int WPER = (WCHUNKS + 255) / 256;
-    __shared__ __attribute__((aligned(16))) bf16_t halo[kHaloH * kHaloW * CP];
-    __shared__ __attribute__((aligned(16))) bf16_t wtap[2][64 * CP];
-    const int n = blockIdx.z;
-    const int ty0 = blockIdx.y * kLdsTileH, tx0 = blockIdx.x * kLdsTileW;
+    const bf16_t* __restrict__ resid, bf16_t* __restrict__ out, int H, int W, int relu,
+    int tiles_x, int tiles_y, int n_tiles) {
+    constexpr int K = 9 * 64;
+    __shared__ __attribute__((aligned(16))) bf16_t wl[64 * 9 * 64];
+    __shared__ __attribute__((aligned(16))) bf16_t halo[kPHaloPix * 64];
     const int tid = threadIdx.x;
-    // stage the halo: (row, col, chunk) -> 16 bytes, zero outside the image
-    for (int e = tid; e < kHaloH * kHaloW * CHUNKS; e += 256) {
-        const int ch = e % CHUNKS;
-        const int pix = e / CHUNKS;
-        const int hx = pix % kHaloW, hy = pix / kHaloW;
-        const int yy = ty0 + hy - 1, xx = tx0 + hx - 1;
-        bf16x8 v = zero8();
-        if (yy >= 0 && yy < H && xx >= 0 && xx < W)
-            v = *reinterpret_cast<const bf16x8*>(in + ((size_t)(n * H + yy) * W + xx) * CIN + ch * 8);
-        *reinterpret_cast<bf16x8*>(&halo[pix * CP + ch * 8]) = v;
+    // all weights: row (co, tap) of 8 chunks
+    for (int e = tid; e < 64 * 9 * 8; e += kPThreads) {
+        const int c = e & 7, row = e >> 3;  // row = co * 9 + tap
+        const int co = row / 9;
+        *reinterpret_cast<bf16x8*>(&wl[(row * 8 + swz(co, c)) * 8]) =
+            *reinterpret_cast<const bf16x8*>(w + (size_t)co * K + (row - co * 9) * 64 + c * 8);
     }
-    // weights of tap 0
+    bf16x8 pre[kPPer];
+    auto fetch = [&](int tile) {
+        const int tx = tile % tiles_x;
+        const int ty = (tile / tiles_x) % tiles_y;
+        const int n = tile / (tiles_x * tiles_y);
+        const int ty0 = ty * kPTileH, tx0 = tx * kPTileW;
 #pragma unroll
-    for (int p = 0; p < WPER; ++p) {
-        const int e = tid + p * 256;
-        if (e < WCHUNKS) {
-            const int co = e / CHUNKS, ch = e % CHUNKS;
-            *reinterpret_cast<bf16x8*>(&wtap[0][co * CP + ch * 8]) =
-                *reinterpret_cast<const bf16x8*>(w + (size_t)co * K + ch * 8);
+        for (int p = 0; p < kPPer; ++p) {
+            const int e = tid + p * kPThreads;
+            bf16x8 v = zero8();
+            if (e < kPHaloChunks) {
+                const int c = e & 7, pix = e >> 3;
+                const int yy = ty0 + pix / kPHaloW - 1, xx = tx0 + pix % kPHaloW - 1;
+                if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+                    v = *reinterpret_cast<const bf16x8*>(in + ((size_t)(n * H + yy) * W + xx) * 64 + c * 8);
+            }
+            pre[p] = v;
         }
-    }
-    __syncthreads();
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int p = 0; p < kPPer; ++p) {
+            const int e = tid + p * kPThreads;
+            if (e < kPHaloChunks) {
+                const int c = e & 7, pix = e >> 3;
+                *reinterpret_cast<bf16x8*>(&halo[(pix * 8 + swz(pix, c)) * 8]) = pre[p];
+            }
+        }
+    };
     const int lane = tid & 63, wave = tid >> 6;
     const int m = lane & 15, g = lane >> 4;
-    f32x4 acc[4][4];
+    float bias_t[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int t = 0; t < 4; ++t) bias_t[t] = bias ? bias[t * 16 + m] : 0.0f;
+    int tile = blockIdx.x;
+    if (tile < n_tiles) {
+        fetch(tile);
+        store();
+    }
+    __syncthreads();
+    while (tile < n_tiles) {
+        const int next = tile + (int)gridDim.x;
+        f32x4 acc[4][4];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) acc[i][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    // M tile i: tile row 2*wave + (i >> 1), columns 16*(i & 1) .. +15
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[i][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        // the previous tile's stores have completed (their data registers are reused by the
+        // accumulators); the next tile's halo is then in flight while this tile is computed
+        __builtin_amdgcn_s_waitcnt(0);
+        if (next < n_tiles) fetch(next);
 #pragma unroll 1
-    for (int tap = 0; tap < 9; ++tap) {
-        const int buf = tap & 1;
-        bf16x8 wn[WPER];
-        if (tap + 1 < 9) {
+        for (int tap = 0; tap < 9; ++tap) {
+            const int dy = tap / 3, dx = tap % 3;
 #pragma unroll
-            for (int p = 0; p < WPER; ++p) {
-                const int e = tid + p * 256;
-                const int co = e / CHUNKS, ch = e % CHUNKS;
-                if (e < WCHUNKS)
-                    wn[p] = *reinterpret_cast<const bf16x8*>(w + (size_t)co * K + (tap + 1) * CIN + ch * 8);
+            for (int c0 = 0; c0 < 8; c0 += 4) {  // chunk offset of the K step (32 channels)
+                const int ch = c0 + g;
+                bf16x8 b[4], a[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int co = t * 16 + m;
+                    b[t] = *reinterpret_cast<const bf16x8*>(&wl[((co * 9 + tap) * 8 + swz(co, ch)) * 8]);
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int pix = (2 * wave + (i >> 1) + dy) * kPHaloW + 16 * (i & 1) + m + dx;
+                    a[i] = *reinterpret_cast<const bf16x8*>(&halo[(pix * 8 + swz(pix, ch)) * 8]);
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int t = 0; t < 4; ++t)
+                        acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[t], acc[i][t], 0, 0, 0);
             }
         }
-        const int dy = tap / 3, dx = tap % 3;  // halo offsets (0..2)
-        const bf16_t* wt = wtap[buf];
+        __syncthreads();  // every wave is done with this halo: reuse it to stage the results
+        // bias + ReLU, as bfloat16, into stage[pixel][channel] (pixel = row * 32 + column, rows
+        // of 8 16-byte chunks swizzled like the halo)
+        bf16_t* stage = halo;
 #pragma unroll
-        for (int c0 = 0; c0 < CIN; c0 += 32) {
-            bf16x8 b[4], a[4];
-#pragma unroll
-            for (int t = 0; t < 4; ++t)
-                b[t] = *reinterpret_cast<const bf16x8*>(&wt[(t * 16 + m) * CP + c0 + g * 8]);
+        for (int t = 0; t < 4; ++t) {
+            const int co = t * 16 + m;
+            const float bc = bias_t[t];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int hy = 2 * wave + (i >> 1) + dy;
-                const int hx = 16 * (i & 1) + m + dx;
-                a[i] = *reinterpret_cast<const bf16x8*>(&halo[(hy * kHaloW + hx) * CP + c0 + g * 8]);
-            }
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int t = 0; t < 4; ++t)
-                    acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[t], acc[i][t], 0, 0, 0);
-        }
-        if (tap + 1 < 9) {
-#pragma unroll
-            for (int p = 0; p < WPER; ++p) {
-                const int e = tid + p * 256;
-                const int co = e / CHUNKS, ch = e % CHUNKS;
-                if (e < WCHUNKS) *reinterpret_cast<bf16x8*>(&wtap[buf ^ 1][co * CP + ch * 8]) = wn[p];
+                for (int r = 0; r < 4; ++r) {
+                    const int p = (2 * wave + (i >> 1)) * kPTileW + 16 * (i & 1) + 4 * g + r;
+                    float v = acc[i][t][r] + bc;
+                    if (relu) v = fmaxf(v, 0.0f);
+                    stage[(p * 8 + swz(p, co >> 3)) * 8 + (co & 7)] = (bf16_t)v;
+                }
             }
         }
         __syncthreads();
-    }
+        {
+            const int tx = tile % tiles_x;
+            const int ty = (tile / tiles_x) % tiles_y;
+            const int n = tile / (tiles_x * tiles_y);
+            // 16-byte chunks (pixel, 8 channels), in two halves of four per thread: the residual
+            // chunks of a half are all loaded before any is used; residual added in FP32
+            constexpr int kChunks = kPTileH * kPTileW * 8 / kPThreads;
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        const int co = t * 16 + m;
-        const float bc = bias ? bias[co] : 0.0f;
+            for (int h0 = 0; h0 < kChunks; h0 += 4) {
+                bf16x8 rv[4];
+                size_t o[4];
+                bool ok[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int yy = ty0 + 2 * wave + (i >> 1);
-            if (yy >= H) continue;
+                for (int k = 0; k < 4; ++k) {
+                    const int e = tid + (h0 + k) * kPThreads;
+                    const int c = e & 7, p = e >> 3;
+                    const int yy = ty * kPTileH + p / kPTileW, xx = tx * kPTileW + p % kPTileW;
+                    ok[k] = yy < H && xx < W;
+                    o[k] = ((size_t)(n * H + min(yy, H - 1)) * W + min(xx, W - 1)) * 64 + c * 8;
+                    rv[k] = (resid && ok[k]) ? *reinterpret_cast<const bf16x8*>(resid + o[k]) : zero8();
+                }
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int xx = tx0 + 16 * (i & 1) + 4 * g + r;
-                if (xx >= W) continue;
-                float v = acc[i][t][r] + bc;
-                if (relu) v = fmaxf(v, 0.0f);
-                const size_t o = ((size_t)(n * H + yy) * W + xx) * 64 + co;
-                if (resid) v += (float)resid[o];
-                out[o] = (bf16_t)v;
+                for (int k = 0; k < 4; ++k) {
+                    const int e = tid + (h0 + k) * kPThreads;
+                    const int c = e & 7, p = e >> 3;
+                    bf16x8 v = *reinterpret_cast<const bf16x8*>(&stage[(p * 8 + swz(p, c)) * 8]);
+                    if (resid) {
+#pragma unroll
+                        for (int q = 0; q < 8; ++q) v[q] = (bf16_t)((float)v[q] + (float)rv[k][q]);
+                    }
+                    if (ok[k]) *reinterpret_cast<bf16x8*>(out + o[k]) = v;
+                }
             }
         }
+        __syncthreads();  // every wave is done with this halo
+        if (next < n_tiles) store();
+        __syncthreads();
+        tile = next;
     }
 }
 
@@ -376,15 +425,17 @@ int ctr_conv3x3_bf16_forward(const void* in, const void* weight, const float* bi
     const bf16_t* wt = (const bf16_t*)weight;
     const bf16_t* r = (const bf16_t*)residual;
     bf16_t* y = (bf16_t*)out;
-    if (cout == 64 && (cin == 64 || cin == 32) && n <= 65535) {
-        const dim3 grid((unsigned)((w + kLdsTileW - 1) / kLdsTileW), (unsigned)((h + kLdsTileH - 1) / kLdsTileH),
-                        (unsigned)n);
-        if (cin == 64)
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(conv3x3_fwd_lds_kernel<64>), grid, dim3(256), 0, s, x, wt, bias,
-                               r, y, h, w, relu);
-        else
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(conv3x3_fwd_lds_kernel<32>), grid, dim3(256), 0, s, x, wt, bias,
-                               r, y, h, w, relu);
+    if (cout == 64 && cin == 64) {
+        const int tiles_x = (w + kPTileW - 1) / kPTileW, tiles_y = (h + kPTileH - 1) / kPTileH;
+        const long long n_tiles = (long long)n * tiles_x * tiles_y;
+        if (n_tiles >= 2147483647LL) return CTR_ERR_INVALID;
+        int dev = 0, n_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return CTR_ERR_HIP;
+        const int grid = (int)std::min<long long>(n_tiles, std::max(n_cu, 1));
+        hipLaunchKernelGGL(conv3x3_fwd_persistent_kernel, dim3(grid), dim3(kPThreads), 0, s, x, wt, bias, r,
+                           y, h, w, relu, tiles_x, tiles_y, (int)n_tiles);
     } else if (cin % 32 == 0) {
         hipLaunchKernelGGL(HIP_KERNEL_NAME(conv3x3_fwd_kernel<true>), dim3((unsigned)blocks), dim3(256),
                            0, s, x, wt, bias, r, y, n, h, w, cin, cout, relu);

@@ -153,7 +153,7 @@ def slices_to_input(vol: torch.Tensor, z: torch.Tensor | None = None, in_slices:
 
 
 @torch.no_grad()
-def apply_to_volume(net: ResidualPrior, vol: torch.Tensor, scale: float = 1.0, batch: int = 32) -> torch.Tensor:
+def apply_to_volume(net: ResidualPrior, vol: torch.Tensor, scale: float = 1.0, batch: int = 128) -> torch.Tensor:
     """Run the prior slice by slice over a volume [nz, ny, nx]; inputs are multiplied by ``scale``
     (network units) and the output divided by it."""
     out = torch.empty_like(vol)
