@@ -527,7 +527,7 @@ __global__ void bp_geometry_kernel(ctr_geometry g, const double* __restrict__ an
 // assigned so that each XCD holds contiguous z-chunks, whose windows share detector rows in
 // that XCD's L2.  A window larger than the LDS buffer is gathered from global memory.
 constexpr int kTileX = 16, kTileY = 16;
-constexpr int kWinCap = 1024;          // floats per LDS window buffer
+constexpr int kWinCap = 512;           // floats per LDS window buffer
 constexpr int kMaxUTaps = 3;           // u taps held in registers (more are looped)
 constexpr int kPrefetch = kWinCap / 256;
 #ifndef CTR_BP_ZC
@@ -753,7 +753,7 @@ __global__ void __launch_bounds__(256) dd_backproject_tile_kernel(
     int nx, int ny, int nz, int nu, int nv, int n_views, float x0, float y0, float z0,
     float dx, float dy, float dz, int curved, float scale, float zero_tol, int accumulate,
     int tiles_x, int tiles_y, int tiles_z) {
-    __shared__ float win[2][kWinCap];
+    __shared__ float win[4][kWinCap];
     __shared__ float scan_lds[(kMaxRows + 2) * 256];
     float* scan = scan_lds + threadIdx.x;
     // XCD-aware tile order: workgroup b runs on XCD b % 8; give each XCD a contiguous range
@@ -787,65 +787,81 @@ __global__ void __launch_bounds__(256) dd_backproject_tile_kernel(
 #pragma unroll
     for (int i = 0; i < ZC; ++i) acc[i] = 0.0f;
 
-    int wv0, wvn, wu0, wun;
-    BpView cur = load_view(vt, 0);
-    tile_window(cur, xa, xb, ya, yb, za, zb, curved, nu, nv, wv0, wvn, wu0, wun);
-    float pf[kPrefetch];
-    if (window_mode(wvn, wun) == 1) {
-        for (int e = threadIdx.x; e < wvn * wun; e += 256)
-            win[0][e] = q[(size_t)(wv0 + e / wun) * nu + wu0 + e % wun];
+    // Views are taken in pairs: the windows of views v+2 and v+3 are loaded into registers while
+    // views v and v+1 are computed from two LDS buffers, then stored into the other two buffers
+    // (one barrier per pair), so a window load has two views of work to arrive.
+    auto window_of = [&](int view, int& v0, int& vn, int& u0, int& un) {
+        if (view < n_views) {
+            tile_window(load_view(vt, view), xa, xb, ya, yb, za, zb, curved, nu, nv, v0, vn, u0, un);
+        } else {
+            v0 = 0; vn = 0; u0 = 0; un = 0;
+        }
+    };
+    auto fetch = [&](int view, int v0, int vn, int u0, int un, float (&pf)[kPrefetch]) {
+        if (window_mode(vn, un) != 1) return;
+        const float* qn = q + (size_t)view * nv * nu + (size_t)v0 * nu + u0;
+        // element e = threadIdx.x + 256 p of the window is (row r, column c); step by 256
+        int r = (int)threadIdx.x / un;
+        int c = (int)threadIdx.x - r * un;
+        const int dr = 256 / un, dc = 256 - dr * un;
+#pragma unroll
+        for (int p = 0; p < kPrefetch; ++p) {
+            if (r < vn) pf[p] = qn[(size_t)r * nu + c];
+            r += dr;
+            c += dc;
+            if (c >= un) {
+                c -= un;
+                ++r;
+            }
+        }
+    };
+    auto put = [&](float* buf, int vn, int un, const float (&pf)[kPrefetch]) {
+        if (window_mode(vn, un) != 1) return;
+#pragma unroll
+        for (int p = 0; p < kPrefetch; ++p) {
+            const int e = threadIdx.x + p * 256;
+            if (e < vn * un) buf[e] = pf[p];
+        }
+    };
+    // windows (first row, rows, first column, columns) of the pair in the current buffers
+    int w0[2], wn[2], wu[2], wun[2];
+    for (int k = 0; k < 2; ++k) {
+        window_of(k, w0[k], wn[k], wu[k], wun[k]);
+        if (window_mode(wn[k], wun[k]) == 1) {
+            for (int e = threadIdx.x; e < wn[k] * wun[k]; e += 256)
+                win[k][e] = q[(size_t)k * nv * nu + (size_t)(w0[k] + e / wun[k]) * nu + wu[k] + e % wun[k]];
+        }
     }
     __syncthreads();
-    for (int view = 0; view < n_views; ++view) {
-        const int buf = view & 1;
-        const int mode = window_mode(wvn, wun);
-        // prefetch the next view's window into registers
-        int nv0 = 0, nvn = 0, nu0 = 0, nun = 0;
-        BpView nxt = cur;
-        const bool has_next = view + 1 < n_views;
-        bool nfits = false;
-        if (has_next) {
-            nxt = load_view(vt, view + 1);
-            tile_window(nxt, xa, xb, ya, yb, za, zb, curved, nu, nv, nv0, nvn, nu0, nun);
-            nfits = window_mode(nvn, nun) == 1;
-            if (nfits) {
-                const float* qn = q + (size_t)(view + 1) * nv * nu + (size_t)nv0 * nu + nu0;
-                // element e = threadIdx.x + 256 p of the window is (row r, column c); step by 256
-                int r = (int)threadIdx.x / nun;
-                int c = (int)threadIdx.x - r * nun;
-                const int dr = 256 / nun, dc = 256 - dr * nun;
-#pragma unroll
-                for (int p = 0; p < kPrefetch; ++p) {
-                    if (r < nvn) pf[p] = qn[(size_t)r * nu + c];
-                    r += dr;
-                    c += dc;
-                    if (c >= nun) {
-                        c -= nun;
-                        ++r;
-                    }
-                }
-            }
-        }
-        if (active && mode != 0) {
+    for (int view = 0; view < n_views; view += 2) {
+        const int s = (view >> 1) & 1;
+        int x0n[2], xnn[2], xun[2], xunn[2];  // windows of views v+2, v+3
+        float pf0[kPrefetch], pf1[kPrefetch];
+        window_of(view + 2, x0n[0], xnn[0], xun[0], xunn[0]);
+        window_of(view + 3, x0n[1], xnn[1], xun[1], xunn[1]);
+        fetch(view + 2, x0n[0], xnn[0], xun[0], xunn[0], pf0);
+        fetch(view + 3, x0n[1], xnn[1], xun[1], xunn[1], pf1);
+#pragma unroll 1
+        for (int k = 0; k < 2; ++k) {
+            const int vw = view + k;
+            const int mode = window_mode(wn[k], wun[k]);
+            if (vw >= n_views || !active || mode == 0) continue;
+            const BpView cur = load_view(vt, vw);
             if (mode == 1) {
-                bp_column_view<ZC, true>(cur, win[buf], wun, wv0, wvn, wu0, wun, X, Y, Z0, dz, hx, hy,
-                                         hz, nz_left, curved, scale, zero_tol, nu, nv, scan, acc);
+                bp_column_view<ZC, true>(cur, win[2 * s + k], wun[k], w0[k], wn[k], wu[k], wun[k], X, Y, Z0,
+                                         dz, hx, hy, hz, nz_left, curved, scale, zero_tol, nu, nv, scan, acc);
             } else {
-                bp_column_view<ZC, false>(cur, q + (size_t)view * nv * nu, nu, 0, nv, 0, nu, X, Y, Z0,
-                                          dz, hx, hy, hz, nz_left, curved, scale, zero_tol, nu, nv, scan,
-                                          acc);
+                bp_column_view<ZC, false>(cur, q + (size_t)vw * nv * nu, nu, 0, nv, 0, nu, X, Y, Z0, dz, hx,
+                                          hy, hz, nz_left, curved, scale, zero_tol, nu, nv, scan, acc);
             }
         }
-        if (has_next && nfits) {
-#pragma unroll
-            for (int p = 0; p < kPrefetch; ++p) {
-                const int e = threadIdx.x + p * 256;
-                if (e < nvn * nun) win[buf ^ 1][e] = pf[p];
-            }
-        }
+        put(win[2 * (1 - s)], xnn[0], xunn[0], pf0);
+        put(win[2 * (1 - s) + 1], xnn[1], xunn[1], pf1);
         __syncthreads();
-        cur = nxt;
-        wv0 = nv0; wvn = nvn; wu0 = nu0; wun = nun;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            w0[k] = x0n[k]; wn[k] = xnn[k]; wu[k] = xun[k]; wun[k] = xunn[k];
+        }
     }
     if (!active) return;
 #pragma unroll
