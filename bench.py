@@ -35,6 +35,9 @@ CONFIGS = {
                noise=True, workload="cone-beam FDK 512^3, 720 views, 1024x768 detector"),
     "c2": dict(n=256, voxel=0.5, views=360, nu=384, nv=384, pitch=1.0, ellipsoids=20, seed=2,
                noise=False, workload="cone-beam FDK 256^3, 360 views, 384x384 detector"),
+    "c1": dict(n=256, nz=1, voxel=0.5, views=360, nu=512, nv=1, pitch=0.5, ellipsoids=10, seed=1,
+               noise=False, shape="Curved", filter="ram-lak", ndim=2,
+               workload="2D equiangular fan-beam FBP 256^2, 360 views, 512 bins"),
 }
 
 
@@ -44,10 +47,10 @@ def make_params(cfg, filter_type="shepp-logan"):
         "SourceInit": [0, 1000.0, 0], "DetectorInit": [0, -500.0, 0], "StartAngle": 0,
         "EndAngle": 2 * np.pi, "NumberOfDetectorPixels": [cfg["nu"], cfg["nv"]],
         "DetectorPixelSize": [cfg["pitch"], cfg["pitch"]], "NumberOfViews": cfg["views"],
-        "ImagePixelSpacing": [cfg["voxel"]] * 3, "NumberOfImage": [n, n, n],
+        "ImagePixelSpacing": [cfg["voxel"]] * 3, "NumberOfImage": [n, n, cfg.get("nz", n)],
         "PhantomCenter": [0, 0, 0], "RotationOrigin": [0, 0, 0], "ReconCenter": [0, 0, 0],
-        "Method": "Distance", "FilterType": filter_type, "cutoff": 1, "GPU": 1,
-        "DetectorShape": "Flat", "Pitch": 0, "DetectorOffset": [0, 0],
+        "Method": "Distance", "FilterType": cfg.get("filter", filter_type), "cutoff": 1, "GPU": 1,
+        "DetectorShape": cfg.get("shape", "Flat"), "Pitch": 0, "DetectorOffset": [0, 0],
     }
 
 
@@ -72,6 +75,7 @@ def cpu_baseline(params, q_gpu, geom, dev, sample_views=(0, 1), slab=8):
     from oracle import dd_oracle as O
 
     g = O.Geometry(params)
+    slab = min(slab, g.nz)
     views = list(sample_views)
     # filter: time the oracle on the sample views
     proj_s = q_gpu["raw"][views].cpu().numpy()
@@ -151,7 +155,8 @@ def main(argv=None):
     stream = torch.cuda.current_stream(dev)
 
     t_setup = time.perf_counter()
-    ell = phantom.random_ellipsoids(cfg["ellipsoids"], phantom.half_extent(geom), cfg["seed"])
+    ell = phantom.random_ellipsoids(cfg["ellipsoids"], phantom.half_extent(geom), cfg["seed"],
+                                    ndim=cfg.get("ndim", 3))
     vol = phantom.ellipsoid_volume(geom, ell, device=dev) * phantom.MU_WATER
     # forward projection (timed separately from the FDK step)
     proj = ctr.dd_forward(vol, geom)
