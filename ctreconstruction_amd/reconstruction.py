@@ -18,7 +18,7 @@ import numpy as np
 import torch
 
 from . import projector as P
-from .geometry import ScanGeometry
+from .geometry import ScanGeometry, merged_params
 
 
 class Reconstruction:
@@ -192,3 +192,94 @@ def _lattice(r: Reconstruction, Source, SDD, angle, curved: bool):
                 arr[k] = Source[k] + U * eu[k] + SDD * ew[k] + V * (k == 2)
         out.append(arr)
     return out[0], out[1]
+
+
+# ---------------------------------------------------------------------------------------------
+# Legacy entry points: Forward(filename, params).forward() and Backward(filename, params).backward()
+# ---------------------------------------------------------------------------------------------
+
+def _legacy_params(params: dict) -> dict:
+    """The legacy classes' params (python/Forward.py:443-450, python/Backward.py:692-700) on the
+    ``Reconstruction`` grid.
+
+    Their voxel edge planes are ``(PhantomCenter - n/2 + k) * d - d/2`` with d = (dx, -dy, -dz)
+    (python/Forward.py:538-543) and their backprojection samples voxels at
+    ``(PhantomCenter - n/2 + i) * d`` (python/Backward.py:835-840), i.e. the ``Reconstruction``
+    grid with ``ReconCenter = (PC_x dx - dx/2, -PC_y dy + dy/2, -PC_z dz + dz/2)``.  ``Origin``
+    gives the rotation centre (SAD), the source and detector sit at z = 0."""
+    p = merged_params(params)
+    dx, dy, dz = (float(v) for v in p["ImagePixelSpacing"])
+    pc = [float(v) for v in p.get("PhantomCenter", [0, 0, 0])]
+    p["ReconCenter"] = [pc[0] * dx - dx / 2, -pc[1] * dy + dy / 2, -pc[2] * dz + dz / 2]
+    p["SourceInit"] = [float(p["SourceInit"][0]), float(p["SourceInit"][1]), 0.0]
+    p["DetectorInit"] = [float(p["DetectorInit"][0]), float(p["DetectorInit"][1]), 0.0]
+    p["Pitch"] = 0
+    p["DetectorOffset"] = [0, 0]
+    return p
+
+
+class Forward:
+    """``Forward(filename, params)`` loads the raw float32 volume ``[nz, ny, nx]`` from
+    ``filename``; ``forward()`` sets ``proj [nViews, nv, nu]`` (python/Forward.py:441-571).
+
+    The projection is the distance-driven projector of ``Reconstruction.forward`` on the legacy
+    voxel grid (see ``_legacy_params``); the numerics are those of the main path, not of the
+    legacy module's own kernels."""
+
+    def __init__(self, filename, params, device=None):
+        self.params = params
+        self._r = Reconstruction(_legacy_params(params), device=device)
+        self.image = self.LoadFile(filename, [self._r.nz, self._r.ny, self._r.nx])
+        self.proj = None
+
+    @staticmethod
+    def LoadFile(filename, image_size, dtype=np.float32):
+        return np.fromfile(filename, dtype=dtype).reshape(image_size)
+
+    def SaveProj(self, filename):
+        _host(self.proj).tofile(filename, sep="", format="")
+
+    def forward(self):
+        self._r.image = self.image
+        self._r.forward()
+        self.proj = self._r.proj
+
+
+class Backward:
+    """``Backward(filename, params)`` loads raw float32 projections ``[nViews, nv, nu]``;
+    ``backward()`` filters every view (``FilterType``, ``cutoff``) and backprojects it, times
+    dtheta, on true detector cell edges (python/Backward.py:808-872), setting ``recon
+    [nz, ny, nx]``.  Runs the main path's filter and FDK-weighted DD backprojector (mode
+    "edge") on the legacy voxel grid (see ``_legacy_params``)."""
+
+    def __init__(self, filename, params, device=None):
+        self.params = params
+        self._r = Reconstruction(_legacy_params(params), device=device)
+        self.proj = self.LoadProj(filename, [self._r.nView, self._r.nv, self._r.nu])
+        self.recon = None
+
+    @staticmethod
+    def LoadProj(filename, image_size, dtype=np.float32):
+        return np.fromfile(filename, dtype=dtype).reshape(image_size)
+
+    @staticmethod
+    def LoadFile(filename, image_size, dtype=np.float32):
+        return np.fromfile(filename, dtype=dtype).reshape(image_size)
+
+    def SaveProj(self, filename):
+        _host(self.proj).tofile(filename, sep="", format="")
+
+    def SaveRecon(self, filename):
+        _host(self.recon).tofile(filename, sep="", format="")
+
+    @staticmethod
+    def Filter(N, pixel_size, FilterType, cutoff):
+        return Reconstruction.Filter(N, pixel_size, FilterType, cutoff)
+
+    def backward(self):
+        r = self._r
+        r._method_check()
+        proj = r._to_device(self.proj, r.geom.proj_shape, "proj")
+        q = P.fdk_filter(proj, r.geom)
+        vol = P.dd_backproject(q, r.geom, mode="edge")
+        self.recon = vol if isinstance(self.proj, torch.Tensor) else _host(vol)

@@ -94,3 +94,22 @@ def test_mapped_geometry_reproduces_conebeam_grid(cone_file):
 def test_overrides_reach_the_params(cone_file):
     r = PF.load_recon_params(cone_file, FilterType="hann", DetectorShape="Curved")
     assert r["FilterType"] == "hann" and r["DetectorShape"] == "Curved"
+
+
+def test_legacy_grid_mapping():
+    """Forward/Backward's grid (python/Forward.py:538-543, python/Backward.py:835-840) on the
+    Reconstruction grid: edge planes (PC - n/2 + k) d - d/2 and sample points (PC - n/2 + i) d."""
+    from ctreconstruction_amd.reconstruction import _legacy_params
+    p = {"SourceInit": [0, 1000.0, 0], "DetectorInit": [0, -500.0, 0], "StartAngle": 0, "EndAngle": 2 * math.pi,
+         "NumberOfDetectorPixels": [32, 24], "DetectorPixelSize": [0.8, 0.8], "NumberOfViews": 10,
+         "ImagePixelSpacing": [0.5, 0.6, 0.7], "NumberOfImage": [10, 12, 8], "PhantomCenter": [1.0, -2.0, 0.5],
+         "Origin": [0, 0, 0], "Method": "Distance", "GPU": 1}
+    g = O.Geometry(_legacy_params(p))
+    pc = np.array(p["PhantomCenter"])
+    d = np.array([0.5, -0.6, -0.7])
+    n = np.array([10, 12, 8])
+    for ax, (planes, centres) in enumerate(zip(g.planes(), g.centres())):
+        k = np.arange(n[ax] + 1)
+        np.testing.assert_allclose(planes, (pc[ax] - n[ax] / 2 + k) * d[ax] - d[ax] / 2, rtol=0, atol=1e-12)
+        i = np.arange(n[ax])
+        np.testing.assert_allclose(centres, (pc[ax] - n[ax] / 2 + i) * d[ax], rtol=0, atol=1e-12)
