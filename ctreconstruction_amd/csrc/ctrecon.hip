@@ -684,22 +684,36 @@ __device__ inline void bp_column_view(const BpView& v, const float* __restrict__
         // copy of the total at ks+nr+1 so that a lookup at the top needs no index clamp
         float csum = 0.0f;
         scan[0] = 0.0f;
-        for (int j = 0; j < nr; ++j) {
-            const float* row = src + (size_t)(ks + j - row0) * stride + cbase;
-            float rk = 0.0f;
+        const float* rp = src + (size_t)(ks - row0) * stride + cbase;
+        if (FROM_LDS && !__any(ntaps > kMaxUTaps)) {
+            // the three taps are consecutive window cells (weight 0 past the footprint; the
+            // window buffers are zero-initialised and padded, so such reads stay finite)
+            float* sp = scan;
+            for (int j = 0; j < nr; ++j) {
+                const float rk = fmaf(wu[0], rp[0], fmaf(wu[1], rp[1], wu[2] * rp[2]));
+                csum = fmaf(rk, W, csum);
+                sp += 256;
+                *sp = csum;
+                rp += stride;
+            }
+        } else {
+            for (int j = 0; j < nr; ++j) {
+                float rk = 0.0f;
 #pragma unroll
-            for (int t = 0; t < kMaxUTaps; ++t) rk = fmaf(wu[t], row[min(t, ntaps - 1)], rk);
-            for (int t = kMaxUTaps; t < ntaps; ++t)
-                rk = fmaf(dd_weight(kus + t, ku0, ku1, ulo, uhi, inv_u, zero_tol), row[t], rk);
-            csum = fmaf(rk, W, csum);
-            scan[(j + 1) * 256] = csum;
+                for (int t = 0; t < kMaxUTaps; ++t) rk = fmaf(wu[t], rp[min(t, ntaps - 1)], rk);
+                for (int t = kMaxUTaps; t < ntaps; ++t)
+                    rk = fmaf(dd_weight(kus + t, ku0, ku1, ulo, uhi, inv_u, zero_tol), rp[t], rk);
+                csum = fmaf(rk, W, csum);
+                scan[(j + 1) * 256] = csum;
+                rp += stride;
+            }
         }
         scan[(nr + 1) * 256] = csum;
         // C(v) between row boundaries is linear; clamping v - ks to [0, nr] gives C = 0 below
         // and the total above (rows outside the detector or the window contribute nothing)
         const float fks = (float)ks, fnr = (float)nr;
         auto integral = [&](float x) {
-            const float f = fminf(fmaxf(x, 0.0f), fnr);
+            const float f = __builtin_amdgcn_fmed3f(x, 0.0f, fnr);
             const int jj = (int)f;
             const float a = scan[jj * 256], b = scan[(jj + 1) * 256];
             return fmaf(__builtin_amdgcn_fractf(f), b - a, a);
@@ -753,7 +767,7 @@ __global__ void __launch_bounds__(256) dd_backproject_tile_kernel(
     int nx, int ny, int nz, int nu, int nv, int n_views, float x0, float y0, float z0,
     float dx, float dy, float dz, int curved, float scale, float zero_tol, int accumulate,
     int tiles_x, int tiles_y, int tiles_z) {
-    __shared__ float win[4][kWinCap];
+    __shared__ float win[4][kWinCap + kMaxUTaps - 1];
     __shared__ float scan_lds[(kMaxRows + 2) * 256];
     float* scan = scan_lds + threadIdx.x;
     // XCD-aware tile order: workgroup b runs on XCD b % 8; give each XCD a contiguous range
@@ -823,6 +837,9 @@ __global__ void __launch_bounds__(256) dd_backproject_tile_kernel(
             if (e < vn * un) buf[e] = pf[p];
         }
     };
+    // zero every window buffer once: cells past a window are read with weight 0
+    for (int e = threadIdx.x; e < 4 * (kWinCap + kMaxUTaps - 1); e += 256) (&win[0][0])[e] = 0.0f;
+    __syncthreads();
     // windows (first row, rows, first column, columns) of the pair in the current buffers
     int w0[2], wn[2], wu[2], wun[2];
     for (int k = 0; k < 2; ++k) {
