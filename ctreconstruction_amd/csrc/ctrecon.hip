@@ -528,6 +528,7 @@ __global__ void bp_geometry_kernel(ctr_geometry g, const double* __restrict__ an
 // that XCD's L2.  A window larger than the LDS buffer is gathered from global memory.
 constexpr int kTileX = 16, kTileY = 16;
 constexpr int kWinCap = 512;           // floats per LDS window buffer
+constexpr int kWinRing = 256;          // views in the LDS ring of detector windows (power of 2)
 constexpr int kMaxUTaps = 3;           // u taps held in registers (more are looped)
 constexpr int kPrefetch = kWinCap / 256;
 #ifndef CTR_BP_ZC
@@ -768,6 +769,7 @@ __global__ void __launch_bounds__(256) dd_backproject_tile_kernel(
     float dx, float dy, float dz, int curved, float scale, float zero_tol, int accumulate,
     int tiles_x, int tiles_y, int tiles_z) {
     __shared__ float win[4][kWinCap + kMaxUTaps - 1];
+    __shared__ int4 wring[kWinRing];
     __shared__ float scan_lds[(kMaxRows + 2) * 256];
     float* scan = scan_lds + threadIdx.x;
     // XCD-aware tile order: workgroup b runs on XCD b % 8; give each XCD a contiguous range
@@ -804,13 +806,23 @@ __global__ void __launch_bounds__(256) dd_backproject_tile_kernel(
     // Views are taken in pairs: the windows of views v+2 and v+3 are loaded into registers while
     // views v and v+1 are computed from two LDS buffers, then stored into the other two buffers
     // (one barrier per pair), so a window load has two views of work to arrive.
-    auto window_of = [&](int view, int& v0, int& vn, int& u0, int& un) {
-        if (view < n_views) {
-            tile_window(load_view(vt, view), xa, xb, ya, yb, za, zb, curved, nu, nv, v0, vn, u0, un);
-        } else {
-            v0 = 0; vn = 0; u0 = 0; un = 0;
+    // The windows depend on the tile and the view only: they are worked out cooperatively, one
+    // view per thread, kWinRing/2 views ahead, into a ring in LDS that every thread reads.
+    auto fill_windows = [&](int first, int count) {
+        for (int t = threadIdx.x; t < count; t += 256) {
+            const int view = first + t;
+            int4 w = make_int4(0, 0, 0, 0);
+            if (view < n_views)
+                tile_window(load_view(vt, view), xa, xb, ya, yb, za, zb, curved, nu, nv, w.x, w.y, w.z, w.w);
+            wring[view & (kWinRing - 1)] = w;
         }
     };
+    auto window_of = [&](int view, int& v0, int& vn, int& u0, int& un) {
+        const int4 w = wring[view & (kWinRing - 1)];
+        v0 = w.x; vn = w.y; u0 = w.z; un = w.w;
+    };
+    fill_windows(0, kWinRing);
+    __syncthreads();
     auto fetch = [&](int view, int v0, int vn, int u0, int un, float (&pf)[kPrefetch]) {
         if (window_mode(vn, un) != 1) return;
         const float* qn = q + (size_t)view * nv * nu + (size_t)v0 * nu + u0;
@@ -852,6 +864,9 @@ __global__ void __launch_bounds__(256) dd_backproject_tile_kernel(
     __syncthreads();
     for (int view = 0; view < n_views; view += 2) {
         const int s = (view >> 1) & 1;
+        // refill the half of the ring last used by views view - kWinRing/2 .. view - 1 (read
+        // before the previous barrier) with views view + kWinRing/2 .. view + kWinRing - 1
+        if (view > 0 && (view & (kWinRing / 2 - 1)) == 0) fill_windows(view + kWinRing / 2, kWinRing / 2);
         int x0n[2], xnn[2], xun[2], xunn[2];  // windows of views v+2, v+3
         float pf0[kPrefetch], pf1[kPrefetch];
         window_of(view + 2, x0n[0], xnn[0], xun[0], xunn[0]);
