@@ -527,8 +527,14 @@ __global__ void bp_geometry_kernel(ctr_geometry g, const double* __restrict__ an
 // assigned so that each XCD holds contiguous z-chunks, whose windows share detector rows in
 // that XCD's L2.  A window larger than the LDS buffer is gathered from global memory.
 constexpr int kTileX = 16, kTileY = 16;
-constexpr int kWinCap = 512;           // floats per LDS window buffer
-constexpr int kWinRing = 256;          // views in the LDS ring of detector windows (power of 2)
+#ifndef CTR_BP_WIN_CAP
+#define CTR_BP_WIN_CAP 512
+#endif
+constexpr int kWinCap = CTR_BP_WIN_CAP;  // floats per LDS window buffer
+#ifndef CTR_BP_WIN_RING
+#define CTR_BP_WIN_RING 256
+#endif
+constexpr int kWinRing = CTR_BP_WIN_RING;  // views in the LDS ring of detector windows (power of 2)
 constexpr int kMaxUTaps = 3;           // u taps held in registers (more are looped)
 constexpr int kPrefetch = kWinCap / 256;
 #ifndef CTR_BP_ZC
@@ -598,12 +604,20 @@ __device__ inline int window_mode(int wvn, int wun) {
     return (wvn * wun <= kWinCap) ? 1 : 2;
 }
 
-template <int ZC, bool FROM_LDS>
-__device__ inline void bp_column_view(const BpView& v, const float* __restrict__ src, int stride,
+// Work split of the backprojection (kind): the fast pass (kBpFast) handles the common case
+// only -- LDS window, z chunk on one side of the source (affine footprints), at most kMaxRows
+// rows and kMaxUTaps columns, no weight zeroing -- and reports every (work-item, view) it
+// leaves; the completing pass (kBpRest) does exactly those; kBpAll does everything.  Keeping
+// the rare cases out of the fast pass keeps its register count, and so its occupancy, low.
+constexpr int kBpFast = 0, kBpAll = 1, kBpRest = 2;
+
+// Returns true when KIND == kBpFast leaves this (work-item, view) to the completing pass.
+template <int ZC, bool FROM_LDS, int KIND>
+__device__ inline bool bp_column_view(const BpView& v, const float* __restrict__ src, int stride,
                                       int row0, int nrows, int col0, int ncols, float X, float Y,
                                       float Z0, float dz, float hx, float hy, float hz, int nz_left,
                                       int curved, float scale, float zero_tol, int nu, int nv,
-                                      float* __restrict__ scan, float (&acc)[ZC]) {
+                                      bool window_in_lds, float* __restrict__ scan, float (&acc)[ZC]) {
     const float xr0 = X * v.c + Y * v.s;
     const float yr0 = Y * v.c - X * v.s;
     float smin = 3.0e38f, smax = -3.0e38f, rmin = 3.0e38f, rmax = -3.0e38f;
@@ -625,7 +639,7 @@ __device__ inline void bp_column_view(const BpView& v, const float* __restrict__
     const int ku0 = (int)floorf(ulo), ku1 = (int)floorf(uhi);
     // taps inside the detector and the source window
     const int kus = max(max(ku0, 0), col0), kue = min(min(ku1, nu - 1), col0 + ncols - 1);
-    if (kus > kue) return;
+    if (kus > kue) return false;
     const float inv_u = __builtin_amdgcn_rcpf(uhi - ulo);
     float wu[kMaxUTaps];
 #pragma unroll
@@ -677,16 +691,24 @@ __device__ inline void bp_column_view(const BpView& v, const float* __restrict__
     const int ks = max(max((int)floorf(rlo), 0), row0);
     const int ke = min(min((int)floorf(rhi), nv - 1), row0 + nrows - 1);
     const int nr = ke - ks + 1;
-    if (nr <= 0) return;
+    if (nr <= 0) return false;
     const int cbase = kus - col0;
     const int ntaps = kue - kus + 1;
-    if (zero_tol <= 0.0f && nr <= kMaxRows) {
+    // the cases the fast pass does
+    const bool common = window_in_lds && lin && nr <= kMaxRows && ntaps <= kMaxUTaps && zero_tol <= 0.0f;
+    if constexpr (KIND == kBpFast) {
+        if (!common) return true;
+    }
+    if constexpr (KIND == kBpRest) {
+        if (common) return false;
+    }
+    if (KIND == kBpFast || (zero_tol <= 0.0f && nr <= kMaxRows)) {
         // row integral C at row boundaries ks .. ks+nr in this work-item's LDS column, plus a
         // copy of the total at ks+nr+1 so that a lookup at the top needs no index clamp
         float csum = 0.0f;
         scan[0] = 0.0f;
         const float* rp = src + (size_t)(ks - row0) * stride + cbase;
-        if (FROM_LDS && !__any(ntaps > kMaxUTaps)) {
+        if (KIND == kBpFast || (FROM_LDS && !__any(ntaps > kMaxUTaps))) {
             // the three taps are consecutive window cells (weight 0 past the footprint; the
             // window buffers are zero-initialised and padded, so such reads stay finite)
             float* sp = scan;
@@ -719,7 +741,7 @@ __device__ inline void bp_column_view(const BpView& v, const float* __restrict__
             const float a = scan[jj * 256], b = scan[(jj + 1) * 256];
             return fmaf(__builtin_amdgcn_fractf(f), b - a, a);
         };
-        if (lin) {
+        if (KIND == kBpFast || lin) {
             const float alo = Alo - fks, ahi = Ahi - fks;
 #pragma unroll
             for (int i = 0; i < ZC; ++i) {
@@ -736,14 +758,10 @@ __device__ inline void bp_column_view(const BpView& v, const float* __restrict__
                 acc[i] = fmaf(integral(hi) - integral(lo), __builtin_amdgcn_rcpf(hi - lo), acc[i]);
             }
         }
-        return;
+        return false;
     }
-    float vlo[ZC], vhi[ZC], inv_v[ZC];
-#pragma unroll
-    for (int i = 0; i < ZC; ++i) {
-        foot((float)i, vlo[i], vhi[i]);
-        inv_v[i] = __builtin_amdgcn_rcpf(vhi[i] - vlo[i]);
-    }
+    // explicit per-row weights (zeroing of small weights, or a chunk over more than kMaxRows
+    // rows): rare, so the voxel footprints are recomputed per row rather than held in registers
     for (int k = ks; k <= ke; ++k) {
         const float* row = src + (size_t)(k - row0) * stride + cbase;
         float rk = 0.0f;
@@ -755,19 +773,24 @@ __device__ inline void bp_column_view(const BpView& v, const float* __restrict__
         const float fk = (float)k, fk1 = (float)(k + 1);
 #pragma unroll
         for (int i = 0; i < ZC; ++i) {
-            float w = (fminf(vhi[i], fk1) - fmaxf(vlo[i], fk)) * inv_v[i];
+            float vlo, vhi;
+            foot((float)i, vlo, vhi);
+            float w = (fminf(vhi, fk1) - fmaxf(vlo, fk)) * __builtin_amdgcn_rcpf(vhi - vlo);
             w = (w < zero_tol || w < 0.0f) ? 0.0f : w;
             acc[i] = fmaf(w, rk, acc[i]);
         }
     }
+    return false;
 }
 
-template <int ZC>
+// KIND: kBpFast writes the volume and sets flags[tile][view] for every view it leaves (in
+// part or whole) to kBpRest, which adds those contributions; kBpAll does everything.
+template <int ZC, int KIND>
 __global__ void __launch_bounds__(256) dd_backproject_tile_kernel(
     const float* __restrict__ q, float* __restrict__ vol, const float* __restrict__ vt,
-    int nx, int ny, int nz, int nu, int nv, int n_views, float x0, float y0, float z0,
-    float dx, float dy, float dz, int curved, float scale, float zero_tol, int accumulate,
-    int tiles_x, int tiles_y, int tiles_z) {
+    uint8_t* __restrict__ flags, int nx, int ny, int nz, int nu, int nv, int n_views, float x0,
+    float y0, float z0, float dx, float dy, float dz, int curved, float scale, float zero_tol,
+    int accumulate, int tiles_x, int tiles_y, int tiles_z) {
     __shared__ float win[4][kWinCap + kMaxUTaps - 1];
     __shared__ int4 wring[kWinRing];
     __shared__ float scan_lds[(kMaxRows + 2) * 256];
@@ -780,6 +803,13 @@ __global__ void __launch_bounds__(256) dd_backproject_tile_kernel(
     const int tx = t % tiles_x;
     const int ty = (t / tiles_x) % tiles_y;
     const int tz = t / (tiles_x * tiles_y);
+    uint8_t* __restrict__ tflags = (KIND == kBpAll) ? nullptr : flags + (size_t)t * n_views;
+    if constexpr (KIND == kBpRest) {
+        // nothing left for this tile: done
+        int any = 0;
+        for (int vw = threadIdx.x; vw < n_views; vw += 256) any |= tflags[vw];
+        if (!__syncthreads_or(any)) return;
+    }
     const int lx = threadIdx.x & (kTileX - 1), ly = threadIdx.x / kTileX;
     const int ix = tx * kTileX + lx, iy = ty * kTileY + ly, iz0 = tz * ZC;
     const bool active = (ix < nx) && (iy < ny);
@@ -877,14 +907,32 @@ __global__ void __launch_bounds__(256) dd_backproject_tile_kernel(
         for (int k = 0; k < 2; ++k) {
             const int vw = view + k;
             const int mode = window_mode(wn[k], wun[k]);
-            if (vw >= n_views || !active || mode == 0) continue;
-            const BpView cur = load_view(vt, vw);
-            if (mode == 1) {
-                bp_column_view<ZC, true>(cur, win[2 * s + k], wun[k], w0[k], wn[k], wu[k], wun[k], X, Y, Z0,
-                                         dz, hx, hy, hz, nz_left, curved, scale, zero_tol, nu, nv, scan, acc);
+            if (vw >= n_views || mode == 0) continue;
+            if constexpr (KIND == kBpRest) {
+                if (!tflags[vw]) continue;
+            }
+            if constexpr (KIND == kBpFast) {
+                if (mode != 1) {  // window too large for LDS: the whole view is left
+                    if (threadIdx.x == 0) tflags[vw] = 1;
+                    continue;
+                }
+                if (!active) continue;
+                if (bp_column_view<ZC, true, kBpFast>(load_view(vt, vw), win[2 * s + k], wun[k], w0[k], wn[k],
+                                                      wu[k], wun[k], X, Y, Z0, dz, hx, hy, hz, nz_left, curved,
+                                                      scale, zero_tol, nu, nv, true, scan, acc))
+                    tflags[vw] = 1;
             } else {
-                bp_column_view<ZC, false>(cur, q + (size_t)vw * nv * nu, nu, 0, nv, 0, nu, X, Y, Z0, dz, hx,
-                                          hy, hz, nz_left, curved, scale, zero_tol, nu, nv, scan, acc);
+                if (!active) continue;
+                const BpView cur = load_view(vt, vw);
+                if (mode == 1) {
+                    bp_column_view<ZC, true, KIND>(cur, win[2 * s + k], wun[k], w0[k], wn[k], wu[k], wun[k], X, Y,
+                                                   Z0, dz, hx, hy, hz, nz_left, curved, scale, zero_tol, nu, nv,
+                                                   true, scan, acc);
+                } else {
+                    bp_column_view<ZC, false, KIND>(cur, q + (size_t)vw * nv * nu, nu, 0, nv, 0, nu, X, Y, Z0,
+                                                    dz, hx, hy, hz, nz_left, curved, scale, zero_tol, nu, nv,
+                                                    false, scan, acc);
+                }
             }
         }
         put(win[2 * (1 - s)], xnn[0], xunn[0], pf0);
@@ -900,8 +948,16 @@ __global__ void __launch_bounds__(256) dd_backproject_tile_kernel(
     for (int i = 0; i < ZC; ++i) {
         if (i >= nz_left) break;
         const size_t idx = ((size_t)(iz0 + i) * ny + iy) * nx + ix;
-        vol[idx] = accumulate ? vol[idx] + acc[i] : acc[i];
+        if constexpr (KIND == kBpRest)
+            vol[idx] += acc[i];
+        else
+            vol[idx] = accumulate ? vol[idx] + acc[i] : acc[i];
     }
+}
+
+__global__ void __launch_bounds__(256) fill_zero_kernel(uint32_t* __restrict__ p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1436,10 +1492,17 @@ AdjLayout adj_layout(const ctr_geometry* g) {
     return L;
 }
 
+// FDK backprojection workspace: per-view constants, then one flag byte per (tile, view)
+size_t bp_tiles(const ctr_geometry* g) {
+    return (size_t)((g->nx + kTileX - 1) / kTileX) * ((g->ny + kTileY - 1) / kTileY) *
+           ((g->nz + kBpZC - 1) / kBpZC);
+}
+
 size_t ctr_dd_backproject_workspace_bytes(const ctr_geometry* g, int mode) {
     if (check_geometry(g)) return 0;
     if (mode == CTR_BP_ADJOINT) return adj_layout(g).total;
-    return align_up((size_t)g->n_views * kBpViewStride * sizeof(float), 256);
+    return align_up((size_t)g->n_views * kBpViewStride * sizeof(float), 256) +
+           align_up(bp_tiles(g) * g->n_views, 256);
 }
 
 int ctr_dd_backproject(const ctr_geometry* g, const double* angles, const int32_t* branch,
@@ -1494,11 +1557,28 @@ int ctr_dd_backproject(const ctr_geometry* g, const double* angles, const int32_
     const int tiles_z = (g->nz + ZC - 1) / ZC;
     const long long tiles = (long long)tiles_x * tiles_y * tiles_z;
     if (tiles >= 2147483647LL) return CTR_ERR_INVALID;
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(dd_backproject_tile_kernel<ZC>), dim3((unsigned)tiles), dim3(256),
-                       0, s, proj, vol, vt, g->nx, g->ny, g->nz, g->nu, g->nv, g->n_views, x0, y0, z0,
-                       (float)g->dx, (float)(-g->dy), (float)(-g->dz),
-                       g->detector_shape == CTR_DETECTOR_CURVED ? 1 : 0, scale, zero_tol, accumulate,
-                       tiles_x, tiles_y, tiles_z);
+    const int curved = g->detector_shape == CTR_DETECTOR_CURVED ? 1 : 0;
+    const float fdx = (float)g->dx, fdy = (float)(-g->dy), fdz = (float)(-g->dz);
+    if (zero_tol > 0.0f) {  // weight zeroing: the general kernel does every view
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(dd_backproject_tile_kernel<ZC, kBpAll>), dim3((unsigned)tiles),
+                           dim3(256), 0, s, proj, vol, vt, (uint8_t*)nullptr, g->nx, g->ny, g->nz, g->nu,
+                           g->nv, g->n_views, x0, y0, z0, fdx, fdy, fdz, curved, scale, zero_tol, accumulate,
+                           tiles_x, tiles_y, tiles_z);
+        return status_from_hip(hipGetLastError());
+    }
+    // fast pass over the common case, then the (tile, view) pairs it left
+    uint8_t* flags = (uint8_t*)workspace + align_up((size_t)g->n_views * kBpViewStride * sizeof(float), 256);
+    const size_t flag_words = align_up((size_t)tiles * g->n_views, 256) / 4;
+    hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)std::min<size_t>((flag_words + 255) / 256, 65536)),
+                       dim3(256), 0, s, (uint32_t*)flags, flag_words);
+    if ((st = status_from_hip(hipGetLastError()))) return st;
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(dd_backproject_tile_kernel<ZC, kBpFast>), dim3((unsigned)tiles),
+                       dim3(256), 0, s, proj, vol, vt, flags, g->nx, g->ny, g->nz, g->nu, g->nv, g->n_views,
+                       x0, y0, z0, fdx, fdy, fdz, curved, scale, zero_tol, accumulate, tiles_x, tiles_y, tiles_z);
+    if ((st = status_from_hip(hipGetLastError()))) return st;
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(dd_backproject_tile_kernel<ZC, kBpRest>), dim3((unsigned)tiles),
+                       dim3(256), 0, s, proj, vol, vt, flags, g->nx, g->ny, g->nz, g->nu, g->nv, g->n_views,
+                       x0, y0, z0, fdx, fdy, fdz, curved, scale, zero_tol, accumulate, tiles_x, tiles_y, tiles_z);
     return status_from_hip(hipGetLastError());
 }
 
