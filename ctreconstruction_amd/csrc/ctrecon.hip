@@ -528,7 +528,7 @@ __global__ void bp_geometry_kernel(ctr_geometry g, const double* __restrict__ an
 // that XCD's L2.  A window larger than the LDS buffer is gathered from global memory.
 constexpr int kTileX = 16, kTileY = 16;
 #ifndef CTR_BP_WIN_CAP
-#define CTR_BP_WIN_CAP 512
+#define CTR_BP_WIN_CAP 576
 #endif
 constexpr int kWinCap = CTR_BP_WIN_CAP;  // floats per LDS window buffer
 #ifndef CTR_BP_WIN_RING
@@ -536,7 +536,7 @@ constexpr int kWinCap = CTR_BP_WIN_CAP;  // floats per LDS window buffer
 #endif
 constexpr int kWinRing = CTR_BP_WIN_RING;  // views in the LDS ring of detector windows (power of 2)
 constexpr int kMaxUTaps = 3;           // u taps held in registers (more are looped)
-constexpr int kPrefetch = kWinCap / 256;
+constexpr int kPrefetch = (kWinCap + 255) / 256;  // window cells per thread
 #ifndef CTR_BP_ZC
 #define CTR_BP_ZC 16
 #endif
@@ -855,6 +855,9 @@ __global__ void __launch_bounds__(256) dd_backproject_tile_kernel(
     __syncthreads();
     auto fetch = [&](int view, int v0, int vn, int u0, int un, float (&pf)[kPrefetch]) {
         if (window_mode(vn, un) != 1) return;
+        if constexpr (KIND == kBpRest) {  // only the views the fast pass left
+            if (view >= n_views || !tflags[view]) return;
+        }
         const float* qn = q + (size_t)view * nv * nu + (size_t)v0 * nu + u0;
         // element e = threadIdx.x + 256 p of the window is (row r, column c); step by 256
         int r = (int)threadIdx.x / un;
@@ -871,8 +874,11 @@ __global__ void __launch_bounds__(256) dd_backproject_tile_kernel(
             }
         }
     };
-    auto put = [&](float* buf, int vn, int un, const float (&pf)[kPrefetch]) {
+    auto put = [&](float* buf, int view, int vn, int un, const float (&pf)[kPrefetch]) {
         if (window_mode(vn, un) != 1) return;
+        if constexpr (KIND == kBpRest) {
+            if (view >= n_views || !tflags[view]) return;
+        }
 #pragma unroll
         for (int p = 0; p < kPrefetch; ++p) {
             const int e = threadIdx.x + p * 256;
@@ -935,8 +941,8 @@ __global__ void __launch_bounds__(256) dd_backproject_tile_kernel(
                 }
             }
         }
-        put(win[2 * (1 - s)], xnn[0], xunn[0], pf0);
-        put(win[2 * (1 - s) + 1], xnn[1], xunn[1], pf1);
+        put(win[2 * (1 - s)], view + 2, xnn[0], xunn[0], pf0);
+        put(win[2 * (1 - s) + 1], view + 3, xnn[1], xunn[1], pf1);
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
