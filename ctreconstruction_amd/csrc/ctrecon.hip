@@ -2,8 +2,8 @@
 //
 // Kernels (all FP32 arithmetic; FP64 only in the per-view geometry kernels):
 //   fp_geometry_kernel          per-view DD footprint tables, FP64 -> FP32 (index space)
-//   zfast_transpose_kernel      vol[z][y][x] -> z-fastest copies [y][x][z] and [x][y][z]
-//   dd_forward_column_kernel    one wavefront per (view, detector column, 64 rows)
+//   zprefix_kernel              vol[z][y][x] -> z-prefix tables [y][x][k] and [x][y][k]
+//   dd_forward_column_kernel    one wavefront per (view, detector column, 63 rows)
 //   bp_geometry_kernel          per-view backprojection constants, FP64 -> FP32
 //   dd_backproject_tile_kernel  one workgroup per 16 x 16 x ZC voxel tile, windows in LDS
 //   adj_geometry_kernel         per-view constants of the inverse central projection
@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <type_traits>
 
 #include "ctrecon.h"
 
@@ -47,7 +48,7 @@ int check_geometry(const ctr_geometry* g) {
     const double vox = (double)g->nx * g->ny * g->nz;
     const double cells = (double)g->n_views * g->nv * g->nu;
     if (vox >= 2147483647.0 || cells >= 2147483647.0) return CTR_ERR_INVALID;
-    // grid y/z limits: transposes (nz/32, ny), backprojection/adjoint (ny/4, nz/4); the
+    // grid y/z limits: z-prefix tables (ny), backprojection/adjoint (ny/4, nz/4); the
     // forward projector packs transverse cell indices into 24 bits
     if (g->n_views > 65535 || g->nv > 65535 * 4 || g->ny > 65535 || g->nz > 65535 ||
         g->nx >= (1 << 24))
@@ -76,15 +77,16 @@ struct FpLayout {
     size_t tabs_off, rows_off, volt_off, copy_bytes, total;
 };
 
-// z stride of the z-fastest volume copies: two zero cells after every line, so that the three
-// z taps of a footprint can always be read as one 12-byte load
-int zfast_stride(int nz) { return nz + 2; }
+// entries per line of the z-prefix tables: breakpoints k = 0 .. nz and a copy of the total at
+// nz + 1, so that the pair (k, k+1) around any point of [0, nz] is inside the line
+int zprefix_stride(int nz) { return nz + 2; }
 
-// workspace: tables, row table, then the volume twice with z fastest: [y][x][z] (gathered by
-// y-slab views) and [x][y][z] (x-slab views)
+// workspace: tables, row table, then the z-prefix tables of the volume (k fastest) twice:
+// [y][x][k] (gathered by y-slab views) and [x][y][k] (x-slab views)
 FpLayout fp_layout(const ctr_geometry* g) {
     FpLayout L;
-    const size_t vbytes = align_up((size_t)g->nx * g->ny * zfast_stride(g->nz) * 4, 256);
+    // + 1024: a staged segment (up to 168 breakpoints) of a shorter line reads past its end
+    const size_t vbytes = align_up((size_t)g->nx * g->ny * zprefix_stride(g->nz) * sizeof(float) + 1024, 256);
     L.tabs_off = 0;
     L.rows_off = align_up(L.tabs_off + (size_t)fp_view_stride(g->nu) * g->n_views * 4, 256);
     L.volt_off = align_up(L.rows_off + (size_t)(2 * g->nv + 1) * 4, 256);
@@ -190,27 +192,45 @@ __device__ inline float dd_weight(int k, int k0, int k1, float lo, float hi, flo
     return (w < zero_tol) ? 0.0f : w;
 }
 
-// vol[z][y][x] -> v_ys[y][x][z] and v_xs[x][y][z] with z stride nzp (cells nz .. nzp-1 zero):
-// 32 (x) x 32 (z) tiles of one y-plane through LDS, coalesced along x on read and along z on
-// write.
-__global__ void __launch_bounds__(256) zfast_transpose_kernel(const float* __restrict__ in,
-                                                              float* __restrict__ v_ys,
-                                                              float* __restrict__ v_xs, int nx,
-                                                              int ny, int nz, int nzp) {
-    __shared__ float tile[32][33];
-    const int x0 = blockIdx.x * 32, z0 = blockIdx.y * 32, y = blockIdx.z;
-    for (int r = threadIdx.y; r < 32; r += 8) {
-        const int z = z0 + r, x = x0 + threadIdx.x;
-        tile[r][threadIdx.x] = (z < nz && x < nx) ? in[((size_t)z * ny + y) * nx + x] : 0.0f;
-    }
-    __syncthreads();
-    for (int r = threadIdx.y; r < 32; r += 8) {
-        const int x = x0 + r, z = z0 + threadIdx.x;
-        if (x < nx && z < nzp) {
-            const float v = tile[threadIdx.x][r];
-            v_ys[((size_t)y * nx + x) * nzp + z] = v;
-            v_xs[((size_t)x * ny + y) * nzp + z] = v;
+// vol[z][y][x] -> z-prefix tables t_ys[y][x][k] and t_xs[x][y][k], k = 0 .. nzp-1: entry k of
+// a line is P_k = f_0 + ... + f_(k-1) (voxels f_c, 0 for c >= nz), summed in FP64 and stored
+// as FP32, so that the z integral of the line from 0 to c (0 <= c <= nz) is the linear
+// interpolation of P at c.  One workgroup per (y, 32 x-lines): 32 (z) x 32 (x) tiles through
+// LDS, coalesced along x on read and along k on write; lanes 0..31 carry the running sums of
+// the 32 lines.
+__global__ void __launch_bounds__(256) zprefix_kernel(const float* __restrict__ in,
+                                                      float* __restrict__ t_ys,
+                                                      float* __restrict__ t_xs, int nx, int ny,
+                                                      int nz, int nzp) {
+    __shared__ float tile[32][33];  // [z][x]
+    __shared__ float line[32][33];  // [x][z]
+    const int x0 = blockIdx.x * 32, y = blockIdx.y;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    double run = 0.0;  // running sum of line x0 + threadIdx.x (threads 0..31)
+    for (int z0 = 0; z0 < nzp; z0 += 32) {
+        for (int r = ty; r < 32; r += 8) {
+            const int z = z0 + r, x = x0 + tx;
+            tile[r][tx] = (z < nz && x < nx) ? in[((size_t)z * ny + y) * nx + x] : 0.0f;
         }
+        __syncthreads();
+        if (threadIdx.x < 32) {
+            for (int r = 0; r < 32; ++r) {
+                const float f = tile[r][threadIdx.x];
+                line[threadIdx.x][r] = (float)run;
+                run += (double)f;
+            }
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < 32 * 32; e += 256) {
+            const int xl = e >> 5, zl = e & 31;
+            const int x = x0 + xl, z = z0 + zl;
+            if (x < nx && z < nzp) {
+                const float v = line[xl][zl];
+                t_ys[((size_t)y * nx + x) * nzp + z] = v;
+                t_xs[((size_t)x * ny + y) * nzp + z] = v;
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -221,57 +241,81 @@ __global__ void __launch_bounds__(256) zfast_transpose_kernel(const float* __res
 // are in index space (plane k at k), so the same code serves increasing (x) and decreasing
 // (y, z) planes.
 //
-// One wavefront per (view, detector column, 64 rows).  All rays of a
-// column share their transverse footprint at every slab (the U-edge lines depend on the column
-// only), so the transverse overlaps, the slab test and the transverse addresses are uniform
-// across the wavefront: lane L works them out for slab n0 + L of a batch of 64 slabs, and the
-// wavefront then walks the batch reading them back from lane q as scalars.  Each lane adds its
-// row's z footprint, c = nu * tt(n) + gamma for the row's two edge slopes nu, whose width
-// |nu1 - nu2| |tt(n)| factors into a per-lane constant and a per-slab scalar, so no per-lane
-// reciprocal is taken per slab.  The volume is read from a z-fastest copy, so the lanes' taps
-// are consecutive in memory.  Overlaps below zero_tol times the footprint width are dropped and
-// a single-cell footprint has weight 1, as in the reference.
+// One wavefront per (view, detector column, kColRows rows).  All rays of a column share their
+// transverse footprint at every slab (the U-edge lines depend on the column only), so the
+// transverse overlaps, the slab test and the transverse addresses are uniform across the
+// wavefront: lane L works them out for slab n0 + L of a batch of 64 slabs, and the wavefront
+// then walks the batch reading them back from lane q as scalars.
+//
+// z direction: the footprint of row k at a slab is [c_k, c_k+1] with c_e = nu_e tt + gamma on
+// the row edges e, and its overlap-weighted sum over a z line is the line's z integral
+// I(c_k+1) - I(c_k), interpolated in the z-prefix table (one 8-byte lookup per edge and line,
+// the two breakpoints around the edge).  Lane
+// L owns edge e0 + L of kColRows + 1 = 64 consecutive edges and sums, in FP64 over the slabs,
+// H_e = sum_slab sum_a w_a I_a(c_e) with the transverse weights w_a pre-divided by wa * tt; row
+// k is then (H_k+1 - H_k) / (nu_k+1 - nu_k), scaled by the ray's length factor.  The width
+// |c_k+1 - c_k| = |nu_k+1 - nu_k| |tt| is the reference's normaliser; a footprint inside one
+// cell gets weight 1 (up to rounding) as in the reference, and cells outside the volume are
+// cut by clamping c_e to [0, nz].  Transverse overlaps below zero_tol times the footprint width
+// are dropped as in the reference; z overlaps are not split per cell, so the reference's
+// per-cell zeroing of z weights below zero_tol (1e-6 of the footprint) has no counterpart.
 constexpr int kColWaves = 4;
+constexpr int kColRows = 63;  // detector rows per wavefront (64 lanes = its 64 row edges)
+#ifndef CTR_FP_UNROLL
+#define CTR_FP_UNROLL 2
+#endif
+constexpr int kFpUnroll = CTR_FP_UNROLL;  // slabs whose segment loads are in flight together
+// z segment of a line staged per slab: 21 lanes x 4 breakpoints per load; the wavefront's
+// three lines arrive in one or two coalesced 16-byte-per-lane loads
+constexpr int kSegLanes = 21;
+constexpr int kSeg = 4 * kSegLanes;
 // Views per block of the workgroup order: consecutive workgroups on one XCD take kViewBlock
-// adjacent views of one column group, then the next column group, so the workgroups resident
-// on an XCD at one time trace nearby rays and share volume lines in its L2.
+// adjacent views of one column group, then the XCD's next column group, so the workgroups
+// resident on an XCD at one time trace nearby rays and share volume lines in its L2.
 #ifndef CTR_FP_VIEW_BLOCK
 #define CTR_FP_VIEW_BLOCK 32
 #endif
 constexpr int kViewBlock = CTR_FP_VIEW_BLOCK;
 
-// three consecutive z cells of a z-fastest line (4-byte aligned)
-struct __attribute__((packed, aligned(4))) ZTaps {
-    float a, b, c;
+// two consecutive breakpoints of a z-prefix line
+struct PTaps {
+    float a, b;
 };
+typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 
 __device__ inline float lane_f(float v, int q) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), q));
 }
 
 __global__ void __launch_bounds__(256) dd_forward_column_kernel(
-    const float* __restrict__ v_ys, const float* __restrict__ v_xs, float* __restrict__ proj,
+    const float* __restrict__ t_ys, const float* __restrict__ t_xs, float* __restrict__ proj,
     const float* __restrict__ tabs, const float* __restrict__ rows,
     const int32_t* __restrict__ branch, int nx, int ny, int nz, int nzp, int nu, int nv, int stride,
-    float zero_tol, int n_views, int col_groups, int n_vblocks, int n_blocks) {
+    float zero_tol, int n_views, int col_groups, int n_vblocks) {
+    // workgroup b runs on XCD b % 8, which takes the column groups cg = xcd + 8 c: every XCD
+    // gets an equal share of every band of detector rows (the rows outside the volume's shadow
+    // are cheap).  On one XCD the local index runs over (row block, view block, c, view in
+    // block), fastest last.
     const int b = blockIdx.x;
-    const int xcd = b & 7, local = b >> 3, qd = n_blocks >> 3, rm = n_blocks & 7;
-    const int t = (xcd < rm) ? xcd * (qd + 1) + local : rm * (qd + 1) + (xcd - rm) * qd + local;
-    const int vi = t % kViewBlock;
-    const int r1 = t / kViewBlock;
-    const int cg = r1 % col_groups;
-    const int r2 = r1 / col_groups;
+    const int xcd = b & 7, l = b >> 3;
+    const int cg8 = (col_groups + 7) >> 3;
+    const int vi = l % kViewBlock;
+    const int r1 = l / kViewBlock;
+    const int cg = (r1 % cg8) * 8 + xcd;
+    const int r2 = r1 / cg8;
     const int view = (r2 % n_vblocks) * kViewBlock + vi;
     const int rb = r2 / n_vblocks;
-    if (view >= n_views) return;
+    if (view >= n_views || cg >= col_groups) return;
     const int lane = threadIdx.x & 63;
     // wave-uniform: lets the compiler keep column data and addresses in scalar registers
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     const int j = cg * kColWaves + wave;
     if (j >= nu) return;
-    const int k = rb * 64 + lane;
-    const bool active = k < nv;
-    const int kk = active ? k : nv - 1;
+    const int e0 = rb * kColRows;
+    const int k = e0 + lane;  // this lane's row (lanes 0 .. kColRows-1) and edge
+    const bool active = lane < kColRows && k < nv;
+    const int kk = min(k, nv - 1);
     const size_t out_idx = ((size_t)view * nv + kk) * nu + j;
     const int br = branch[view];
     if (br != CTR_BRANCH_X && br != CTR_BRANCH_Y) {
@@ -279,56 +323,75 @@ __global__ void __launch_bounds__(256) dd_forward_column_kernel(
         return;
     }
     const bool xs = (br == CTR_BRANCH_X);
-    const float* __restrict__ src = xs ? v_xs : v_ys;
+    const float* __restrict__ src = xs ? t_xs : t_ys;
     const int ns = xs ? nx : ny;
     const int nt = xs ? ny : nx;
-    const size_t slab_stride = (size_t)nt * nzp;
+    // byte offsets inside a batch of 64 slabs fit in 31 bits (host check)
+    const int line_bytes = 4 * nzp;
+    const int slab_bytes = line_bytes * nt;
+    const char* __restrict__ src_bytes = reinterpret_cast<const char*>(src);
     const float* T = tabs + (size_t)view * stride;
     const float a_al = T[j], b_al = T[j + 1];
     const float a_be = T[nu + 1 + j], b_be = T[nu + 2 + j];
     const float T0 = T[2 * nu + 2 + j], dT = T[3 * nu + 2 + j];
     const float sec = T[4 * nu + 2 + j], inv = T[5 * nu + 2 + j];
     const float gam = T[6 * nu + 2];
-    const float nu1 = rows[kk + 1], nu2 = rows[kk];
+    const float nue = rows[min(k, nv)];  // this lane's edge slope
     const float vc = rows[nv + 1 + kk];
     const float s2 = vc * inv;
-    const float dnu = fabsf(nu1 - nu2);  // z footprint width per unit |tt|
-    const float Lk = sec * sqrtf(1.0f + s2 * s2) / dnu;
-    const float tol_c = zero_tol * dnu;
+    const float Lk = sec * sqrtf(1.0f + s2 * s2) / (rows[kk + 1] - rows[kk]);
     const float fnz = (float)nz, fnt = (float)nt;
-    // z extent of the whole wavefront: its rows' edge slopes run from nu_first to nu_last, and
-    // no row's footprint is wider than dnu_max |tt|
-    const int e_first = rb * 64, e_last = min(rb * 64 + 64, nv);
-    const float nu_first = rows[e_first], nu_last = rows[e_last];
-    float dnu_max = dnu;
+    // z extent of the wavefront's edges
+    const float nu_first = rows[e0], nu_last = rows[min(e0 + kColRows, nv)];
+    // gamma in a vector register: the edge fma then reads one scalar operand (tt)
+    float gam_v;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(gam_v) : "s"(gam));
+    double H = 0.0;
+    // the wavefront's LDS segment buffer.  With NL loads per slab (QL = NL kSegLanes quads per
+    // line), quad g = 64 i + L (load i, lane L) holds breakpoints 4 j .. 4 j + 3 of line l at
+    // bytes 16 (QL l + j); quads past the three lines repeat the last one
+    __shared__ uint4 segs[kColWaves][128];
+    uint4* __restrict__ seg = segs[wave];
+    const char* seg_bytes = reinterpret_cast<const char*>(seg);
+    int seg_l[2][2], seg_j16[2][2];
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) dnu_max = fmaxf(dnu_max, __shfl_xor(dnu_max, o));
-    dnu_max = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(dnu_max)));
-    float acc = 0.0f;
-    // z overlaps of this lane's footprint with cells kcs .. kcs+2 and the byte offset of cell
-    // kcs (taps past the footprint read the next cells, or the zero padding, with weight 0; a
-    // footprint outside the volume reads from cell nz - 1 with all weights 0)
-    auto z_taps = [&](float tt, float& o0, float& o1, float& o2, unsigned& v0) {
-        const float c1 = fmaf(nu1, tt, gam), c2 = fmaf(nu2, tt, gam);
-        const float clo = fminf(c1, c2), chi = fmaxf(c1, c2);
-        const int kcs = max((int)floorf(clo), 0);
-        const float tl = tol_c * fabsf(tt);  // zero_tol * wc
-        const float lo = fmaxf(clo, 0.0f), hi = fminf(chi, fnz);
-        const float s1 = (float)kcs + 1.0f;
-        o0 = fminf(hi, s1) - lo;
-        o1 = fminf(hi, s1 + 1.0f) - s1;
-        o2 = hi - (s1 + 1.0f);
-        o0 = (o0 >= tl) ? o0 : 0.0f;
-        o1 = (o1 >= tl) ? o1 : 0.0f;
-        o2 = (o2 >= tl) ? o2 : 0.0f;
-        v0 = 4u * (unsigned)min(kcs, nz - 1);
+    for (int nl = 1; nl <= 2; ++nl) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int ql = nl * kSegLanes, g = 64 * i + lane;
+            const int l = min(g / ql, 2);
+            seg_l[nl - 1][i] = l;
+            seg_j16[nl - 1][i] = 16 * min(g - ql * l, ql - 1);
+        }
+    }
+    // this lane's edge at the slab with fraction tt: its table entry's byte offset in a line
+    // and the fractional part of the clamped edge
+    auto edge = [&](float tt, unsigned& off, float& fr) {
+        const float c = __builtin_amdgcn_fmed3f(fmaf(nue, tt, gam_v), 0.0f, fnz);
+        const float fl = floorf(c);
+        fr = c - fl;
+        off = 4u * (unsigned)fl;
     };
     for (int n0 = 0; n0 < ns; n0 += 64) {
+        // the batch's slabs through one buffer resource: a line is addressed by a scalar byte
+        // offset, the lane's breakpoint pair by a vector one
+        const char* batch = src_bytes + (size_t)n0 * slab_bytes;
+        const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)batch, 0, (int)((size_t)min(64, ns - n0) * slab_bytes), 0x00020000);
+        auto pair = [&](unsigned off, int line_off) {
+            const v2u v = __builtin_amdgcn_raw_buffer_load_b64(rsrc, (int)off, line_off, 0);
+            PTaps t;
+            t.a = __uint_as_float(v.x);
+            t.b = __uint_as_float(v.y);
+            return t;
+        };
         // lane L: transverse footprint of slab n0 + L.  Mode 0: misses the volume (transverse,
-        // or in z for every row of the wavefront), 1: at most three cells in both directions
-        // (weights pre-divided by wa |tt|), 2: wider, explicit loop.
+        // or in z for every edge of the wavefront); 1 and 3: at most three cells (weights
+        // pre-divided by wa tt, first line at byte offset b_o0 in the batch) and the edges'
+        // breakpoints inside a segment staged with one (1) or two (3) loads (its start and da
+        // in b_kd); 2: otherwise, explicit loop.
         float b_tt = 1.0f, b_w0 = 0.0f, b_w1 = 0.0f, b_w2 = 0.0f, b_alo = 0.0f, b_ahi = 0.0f;
-        int b_kas = 0, b_da = 0, b_mode = 0;
+        int b_o0 = 0, b_kd = 0, b_mode = 0;
         {
             const int n = n0 + lane;
             if (n < ns) {
@@ -354,106 +417,118 @@ __global__ void __launch_bounds__(256) dd_forward_column_kernel(
                     o0 = (o0 >= tl) ? o0 : 0.0f;
                     o1 = (o1 >= tl) ? o1 : 0.0f;
                     o2 = (o2 >= tl) ? o2 : 0.0f;
-                    const float r = __builtin_amdgcn_rcpf(wa * fabsf(tt));
+                    const float r = __builtin_amdgcn_rcpf(wa * tt);
                     b_w0 = o0 * r;
                     b_w1 = o1 * r;
                     b_w2 = o2 * r;
-                    b_kas = kas;
-                    b_da = da;
-                    // z footprints no wider than 1.99 cells touch at most three cells
-                    b_mode = (da <= 2 && dnu_max * fabsf(tt) <= 1.99f) ? 1 : 2;
+                    b_o0 = lane * slab_bytes + kas * line_bytes;
+                    // breakpoints of the wavefront's edges: floor of the clamped end edges, the
+                    // same arithmetic as each lane's (the edges are monotone in the lane)
+                    const int kf = (int)floorf(__builtin_amdgcn_fmed3f(fmaf(nu_first, tt, gam), 0.0f, fnz));
+                    const int kl = (int)floorf(__builtin_amdgcn_fmed3f(fmaf(nu_last, tt, gam), 0.0f, fnz));
+                    const int kmin = min(kf, kl), span = max(kf, kl) - kmin;
+                    const int nl = (span <= kSeg - 2) ? 1 : (span <= 2 * kSeg - 2) ? 2 : 0;
+                    const int kseg = min(kmin, max(nzp - nl * kSeg, 0));
+                    // quads per line that hold breakpoints kseg .. kmax + 1 (<= 21 nl)
+                    const int nq = (max(kf, kl) - kseg + 5) >> 2;
+                    b_kd = kseg | (min(da, 2) << 16) | (nq << 18);  // kseg <= nz < 2^16 (host check)
+                    b_mode = (da > 2 || nl == 0) ? 2 : (nl == 1) ? 1 : 3;
                 }
             }
         }
-        // mode-1 slabs, software-pipelined: the taps of the next slab are loaded while the
+        // mode-1 slabs, software-pipelined: the lookups of the next slab are loaded while the
         // current one is summed
-        uint64_t fast = __ballot(b_mode == 1);
-        if (fast) {
-            float f[9], o[3], w[3];
-            auto issue = [&](int q, float (&ff)[9], float (&oo)[3], float (&ww)[3]) {
+        // mode-1 and mode-3 slabs
+        auto walk = [&](auto nl, uint64_t mask) {
+            constexpr int NL = decltype(nl)::value;  // loads per slab
+            constexpr int QL = NL * kSegLanes;      // quads per line
+            struct Set {
+                v4u d[NL];
+                float w[3], fr;
+                unsigned rel;  // byte offset of the lane's breakpoint pair in a line's segment
+            };
+            auto issue = [&](int q, Set& S) {
                 const float tt = lane_f(b_tt, q);
-                unsigned v0;
-                z_taps(tt, oo[0], oo[1], oo[2], v0);
-                const int kas = __builtin_amdgcn_readlane(b_kas, q);
-                const int da = __builtin_amdgcn_readlane(b_da, q);
-                const float* slab = src + (size_t)(n0 + q) * slab_stride;
-                const char* r0 = reinterpret_cast<const char*>(slab + (size_t)kas * nzp);
-                const char* r1 = reinterpret_cast<const char*>(slab + (size_t)(kas + min(1, da)) * nzp);
-                const char* r2 = reinterpret_cast<const char*>(slab + (size_t)(kas + min(2, da)) * nzp);
-                const ZTaps t0 = *reinterpret_cast<const ZTaps*>(r0 + v0);
-                const ZTaps t1 = *reinterpret_cast<const ZTaps*>(r1 + v0);
-                const ZTaps t2 = *reinterpret_cast<const ZTaps*>(r2 + v0);
-                ff[0] = t0.a; ff[1] = t0.b; ff[2] = t0.c;
-                ff[3] = t1.a; ff[4] = t1.b; ff[5] = t1.c;
-                ff[6] = t2.a; ff[7] = t2.b; ff[8] = t2.c;
-                ww[0] = lane_f(b_w0, q);
-                ww[1] = lane_f(b_w1, q);
-                ww[2] = lane_f(b_w2, q);
+                unsigned off;
+                edge(tt, off, S.fr);
+                const int kd = __builtin_amdgcn_readlane(b_kd, q);
+                const int kseg = kd & 0xffff, da = (kd >> 16) & 3, nq16 = (kd >> 18) << 4;
+                const int base = __builtin_amdgcn_readlane(b_o0, q) + 4 * kseg;
+                // quads past the breakpoints in use get an offset beyond the buffer: the range
+                // check returns 0 for them without a memory access
+#pragma unroll
+                for (int i = 0; i < NL; ++i)
+                    S.d[i] = __builtin_amdgcn_raw_buffer_load_b128(
+                        rsrc, seg_j16[NL - 1][i] < nq16 ? min(seg_l[NL - 1][i], da) * line_bytes + seg_j16[NL - 1][i]
+                                                        : 0x7ffffff0, base, 0);
+                S.rel = off - 4u * (unsigned)kseg;
+                S.w[0] = lane_f(b_w0, q);
+                S.w[1] = lane_f(b_w1, q);
+                S.w[2] = lane_f(b_w2, q);
             };
-            auto consume = [&](const float (&ff)[9], const float (&oo)[3], const float (&ww)[3]) {
-                const float col0 = fmaf(oo[0], ff[0], fmaf(oo[1], ff[1], oo[2] * ff[2]));
-                const float col1 = fmaf(oo[0], ff[3], fmaf(oo[1], ff[4], oo[2] * ff[5]));
-                const float col2 = fmaf(oo[0], ff[6], fmaf(oo[1], ff[7], oo[2] * ff[8]));
-                acc = fmaf(ww[0], col0, fmaf(ww[1], col1, fmaf(ww[2], col2, acc)));
+            auto consume = [&](const Set& S) {
+#pragma unroll
+                for (int i = 0; i < NL; ++i) seg[64 * i + lane] = make_uint4(S.d[i].x, S.d[i].y, S.d[i].z, S.d[i].w);
+                const PTaps p0 = *reinterpret_cast<const PTaps*>(seg_bytes + S.rel);
+                const PTaps p1 = *reinterpret_cast<const PTaps*>(seg_bytes + 16 * QL + S.rel);
+                const PTaps p2 = *reinterpret_cast<const PTaps*>(seg_bytes + 32 * QL + S.rel);
+                const float i0 = fmaf(S.fr, p0.b - p0.a, p0.a);
+                const float i1 = fmaf(S.fr, p1.b - p1.a, p1.a);
+                const float i2 = fmaf(S.fr, p2.b - p2.a, p2.a);
+                H += (double)fmaf(S.w[0], i0, fmaf(S.w[1], i1, S.w[2] * i2));
             };
-            // two register sets used in turn, so no copy waits on the loads in flight
-            float g[9], p[3], x[3];
-            int q = __builtin_ctzll(fast);
-            fast &= fast - 1;
-            issue(q, f, o, w);
-            while (true) {
-                if (fast) {
-                    q = __builtin_ctzll(fast);
-                    fast &= fast - 1;
-                    issue(q, g, p, x);
-                    consume(f, o, w);
-                } else {
-                    consume(f, o, w);
-                    break;
+            // kFpUnroll slabs per round: their loads go out together, then they are summed in
+            // order; no load result lives across rounds, so nothing waits to be copied.  Every
+            // round loads and sums kFpUnroll slabs (past the last one it repeats that slab with
+            // weight 0), so the loads in flight, and with them each wait, are the same on all
+            // paths
+            while (mask) {
+                Set S[kFpUnroll];
+                int q = 0;
+#pragma unroll
+                for (int u = 0; u < kFpUnroll; ++u) {
+                    const bool more = mask != 0;
+                    if (more) {
+                        q = __builtin_ctzll(mask);
+                        mask &= mask - 1;
+                    }
+                    issue(q, S[u]);
+                    if (!more) S[u].w[0] = S[u].w[1] = S[u].w[2] = 0.0f;
                 }
-                if (fast) {
-                    q = __builtin_ctzll(fast);
-                    fast &= fast - 1;
-                    issue(q, f, o, w);
-                    consume(g, p, x);
-                } else {
-                    consume(g, p, x);
-                    break;
-                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int u = 0; u < kFpUnroll; ++u) consume(S[u]);
             }
-        }
-        // mode-2 slabs: explicit loops over every cell of the footprint
+        };
+        const uint64_t m1 = __ballot(b_mode == 1);
+        if (m1) walk(std::integral_constant<int, 1>{}, m1);
+        const uint64_t m3 = __ballot(b_mode == 3);
+        if (m3) walk(std::integral_constant<int, 2>{}, m3);
+        // mode-2 slabs: explicit loop over every transverse cell of the footprint
         uint64_t wide = __ballot(b_mode == 2);
         while (wide) {
             const int q = __builtin_ctzll(wide);
             wide &= wide - 1;
             const float tt = lane_f(b_tt, q);
-            const float att = fabsf(tt);
-            const float* slab = src + (size_t)(n0 + q) * slab_stride;
-            const float c1 = fmaf(nu1, tt, gam), c2 = fmaf(nu2, tt, gam);
-            const float clo = fminf(c1, c2), chi = fmaxf(c1, c2);
-            const int kcs = max((int)floorf(clo), 0), kce = min((int)floorf(chi), nz - 1);
-            const float tl = tol_c * att;  // zero_tol * wc
+            unsigned off;
+            float fr;
+            edge(tt, off, fr);
             const float alo = lane_f(b_alo, q), ahi = lane_f(b_ahi, q);
             const float wa = ahi - alo;
             const int kas = max((int)floorf(alo), 0), kae = min((int)floorf(ahi), nt - 1);
-            if (kcs > kce) continue;
             float sum = 0.0f;
             for (int ka = kas; ka <= kae; ++ka) {
                 const float ov = fminf(ahi, (float)(ka + 1)) - fmaxf(alo, (float)ka);
                 if (ov < zero_tol * wa) continue;
-                const float* line = slab + (size_t)ka * nzp;
-                float col = 0.0f;
-                for (int kc = kcs; kc <= kce; ++kc) {
-                    const float p = fminf(chi, (float)(kc + 1)) - fmaxf(clo, (float)kc);
-                    if (p >= tl) col = fmaf(line[kc], p, col);
-                }
-                sum = fmaf(col, ov, sum);
+                const PTaps v = pair(off, q * slab_bytes + ka * line_bytes);
+                sum = fmaf(ov, fmaf(fr, v.b - v.a, v.a), sum);
             }
-            acc = fmaf(sum, __builtin_amdgcn_rcpf(wa * att), acc);
+            H += (double)(sum * __builtin_amdgcn_rcpf(wa * tt));
         }
     }
-    if (active) proj[out_idx] = acc * Lk;
+    // row k = edges k and k+1: the next lane's sum
+    const double Hn = __shfl_down(H, 1);
+    if (active) proj[out_idx] = (float)(Hn - H) * Lk;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -977,7 +1052,8 @@ __global__ void __launch_bounds__(256) fill_zero_kernel(uint32_t* __restrict__ p
 // slab n are found by inverting the central projection through the source (edge coordinate
 // of the ray through a point), with a rounding margin; every candidate then gets the
 // forward projector's own overlap weights from the same FP64-built tables, so a tap carries
-// y * L * o_a * o_c / (w_a * w_c) exactly as the forward sum carries it.
+// y * L * o_a * o_c / (w_a * w_c) as the forward sum carries it (transverse overlaps below
+// 1e-6 of the footprint dropped, z overlaps kept, as there).
 constexpr int kAdjViewStride = 16;
 constexpr int kAdjZChunk = 8;
 
@@ -1101,7 +1177,6 @@ __global__ void __launch_bounds__(256) dd_adjoint_kernel(
             // normalisation as the forward projector forms it: the z width |nu_k+1 - nu_k| |t|
             // is split into |t| (here, with the transverse width) and a per-row factor (in y')
             const float wnorm = oa * __builtin_amdgcn_rcpf(wa * fabsf(t));
-            const float tl = fabsf(t);
             // rows whose edges bracket the chunk [iz0, iz0 + nzc]: nu_e = (c - gam) / t,
             // e = (nu_e - nuA) / nuB
             const float r1 = ((fz0 - gam) * rt - A[11]) * A[12];
@@ -1115,13 +1190,12 @@ __global__ void __launch_bounds__(256) dd_adjoint_kernel(
                 const float c1 = fmaf(rows[k + 1], t, gam);
                 const float c2 = fmaf(rows[k], t, gam);
                 const float clo = fminf(c1, c2), chi = fmaxf(c1, c2);
-                const float thr = tol * fabsf(rows[k + 1] - rows[k]) * tl;
                 const float yk = yv[(size_t)k * nu + j];
 #pragma unroll
                 for (int i = 0; i < kAdjZChunk; ++i) {
                     const float fkc = fz0 + (float)i;
                     const float oc = fminf(chi, fkc + 1.0f) - fmaxf(clo, fkc);
-                    const float w = (oc > 0.0f && oc >= thr) ? oc : 0.0f;
+                    const float w = (oc > 0.0f) ? oc : 0.0f;
                     csum[i] = fmaf(yk, w, csum[i]);
                 }
             }
@@ -1404,29 +1478,32 @@ int ctr_dd_forward(const ctr_geometry* g, const double* angles, const int32_t* b
     if (!angles || !branch || !vol || !proj || !workspace) return CTR_ERR_INVALID;
     const FpLayout L = fp_layout(g);
     if (workspace_bytes < L.total) return CTR_ERR_WORKSPACE;
+    // the projector addresses a batch of 64 slabs of the z-prefix tables with 31-bit offsets
+    if (64.0 * 4.0 * zprefix_stride(g->nz) * std::max(g->nx, g->ny) >= 2147483647.0)
+        return CTR_ERR_UNSUPPORTED;
     hipStream_t s = (hipStream_t)stream;
     char* ws = (char*)workspace;
     float* tabs = (float*)(ws + L.tabs_off);
     float* rows = (float*)(ws + L.rows_off);
-    float* v_ys = (float*)(ws + L.volt_off);
-    float* v_xs = (float*)(ws + L.volt_off + L.copy_bytes);
+    float* t_ys = (float*)(ws + L.volt_off);
+    float* t_xs = (float*)(ws + L.volt_off + L.copy_bytes);
     const int stride = fp_view_stride(g->nu);
     const int gx = (std::max(g->nu + 1, g->nv + 1) + 255) / 256;
     hipLaunchKernelGGL(fp_geometry_kernel, dim3(gx, g->n_views), dim3(256), 0, s, *g, angles, branch,
                        tabs, rows, stride);
     if ((st = status_from_hip(hipGetLastError()))) return st;
-    const int nzp = zfast_stride(g->nz);
-    hipLaunchKernelGGL(zfast_transpose_kernel, dim3((g->nx + 31) / 32, (nzp + 31) / 32, g->ny),
-                       dim3(32, 8), 0, s, vol, v_ys, v_xs, g->nx, g->ny, g->nz, nzp);
+    const int nzp = zprefix_stride(g->nz);
+    hipLaunchKernelGGL(zprefix_kernel, dim3((g->nx + 31) / 32, g->ny), dim3(256), 0, s, vol, t_ys, t_xs,
+                       g->nx, g->ny, g->nz, nzp);
     if ((st = status_from_hip(hipGetLastError()))) return st;
     const int col_groups = (g->nu + kColWaves - 1) / kColWaves;
-    const int row_blocks = (g->nv + 63) / 64;
+    const int row_blocks = (g->nv + kColRows - 1) / kColRows;
     const int n_vblocks = (g->n_views + kViewBlock - 1) / kViewBlock;
-    const long long n_blocks = (long long)col_groups * row_blocks * n_vblocks * kViewBlock;
+    const long long n_blocks = 8LL * ((col_groups + 7) / 8) * row_blocks * n_vblocks * kViewBlock;
     if (n_blocks >= 2147483647LL) return CTR_ERR_INVALID;
-    hipLaunchKernelGGL(dd_forward_column_kernel, dim3((unsigned)n_blocks), dim3(256), 0, s, v_ys, v_xs,
+    hipLaunchKernelGGL(dd_forward_column_kernel, dim3((unsigned)n_blocks), dim3(256), 0, s, t_ys, t_xs,
                        proj, tabs, rows, branch, g->nx, g->ny, g->nz, nzp, g->nu, g->nv, stride, 1e-6f,
-                       g->n_views, col_groups, n_vblocks, (int)n_blocks);
+                       g->n_views, col_groups, n_vblocks);
     return status_from_hip(hipGetLastError());
 }
 
@@ -1522,7 +1599,7 @@ int ctr_dd_backproject(const ctr_geometry* g, const double* angles, const int32_
     if (workspace_bytes < ctr_dd_backproject_workspace_bytes(g, mode)) return CTR_ERR_WORKSPACE;
     hipStream_t s = (hipStream_t)stream;
     if (mode == CTR_BP_ADJOINT) {
-        // transpose of ctr_dd_forward: its weights (and its 1e-6 weight zeroing) by construction
+        // transpose of ctr_dd_forward: its weights (and its 1e-6 transverse weight zeroing) by construction
         if (!branch) return CTR_ERR_INVALID;
         const AdjLayout L = adj_layout(g);
         char* ws = (char*)workspace;

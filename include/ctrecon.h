@@ -91,7 +91,8 @@ int ctr_zero_padded_length(int nu);
  * (python/GPUFuncs_pycuda.py:398-524).  Per-view footprint tables are built on the GPU in
  * FP64; the projector is a gather (one work-item per detector cell, slabs looped in
  * registers, no atomics, deterministic).  `proj` is overwritten.  `vol` and `proj` must not
- * overlap the workspace. */
+ * overlap the workspace.  CTR_ERR_UNSUPPORTED when 256 * (nz + 2) * max(nx, ny) >= 2^31
+ * (e.g. above 2048 x 2048 x 2046). */
 size_t ctr_dd_forward_workspace_bytes(const ctr_geometry* g);
 int ctr_dd_forward(const ctr_geometry* g, const double* angles, const int32_t* branch,
                    const float* vol, float* proj, void* workspace, size_t workspace_bytes,
@@ -124,7 +125,7 @@ int ctr_fdk_filter(const ctr_geometry* g, const double* spectrum, const float* p
  * scale = dtheta (Q = proj * dtheta, :411).  Overlap weights below `zero_tol` are zeroed
  * (0 = never, as the pycuda kernel; 1e-5 = the numba kernels).
  * mode CTR_BP_ADJOINT computes scale * A^T proj, the exact transpose of ctr_dd_forward (its
- * weights and its 1e-6 zeroing; `zero_tol` is ignored); it needs `branch` and has no
+ * weights and its 1e-6 transverse zeroing; `zero_tol` is ignored); it needs `branch` and has no
  * counterpart in the reference, which only has the FDK-weighted backprojector. */
 size_t ctr_dd_backproject_workspace_bytes(const ctr_geometry* g, int mode);
 int ctr_dd_backproject(const ctr_geometry* g, const double* angles, const int32_t* branch,

@@ -87,7 +87,9 @@ def test_adjoint_identity_at_batch_configuration(dev):
 
 
 def test_finite_difference_gradient_of_data_term(dev):
-    # f(x) = ||A x - b||^2 ; autograd gradient (backward = A^T) vs central differences, eps = 1e-3
+    # f(x) = ||A x - b||^2 ; autograd gradient (backward = A^T) vs central differences.  f is
+    # quadratic in x, so a central difference is exact at any step; a large step keeps the FP32
+    # rounding of the projections (relative ~1e-7 of the z line integrals) far below it
     geom = ctr.ScanGeometry(make_params(**C4))
     ell = phantom.random_ellipsoids(8, phantom.half_extent(geom), 401)
     x = (phantom.ellipsoid_volume(geom, ell, device=dev) * 0.5).contiguous()
@@ -103,7 +105,7 @@ def test_finite_difference_gradient_of_data_term(dev):
 
     rng = np.random.default_rng(402)
     idx = [tuple(int(rng.integers(16, n - 16)) for n in geom.vol_shape) for _ in range(64)]
-    eps = 1e-3
+    eps = 0.25
     num, ana = [], []
     for i in idx:
         xp = x0.clone()
