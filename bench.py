@@ -28,7 +28,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-BP_KERNEL = "dd_backproject_tile_kernel<16>"  # the dominant kernel of one FDK step
+# the dominant kernel of one FDK step: the backprojection's common-case pass (the call also
+# launches a flag fill and the remainder pass, both under 1% of it for this workload; the HIP
+# events bracket all three, so the achieved rate is a lower bound for the pass itself)
+BP_KERNEL = "dd_backproject_tile_kernel<16, 0>"
 
 CONFIGS = {
     "c3": dict(n=512, voxel=0.25, views=720, nu=1024, nv=768, pitch=0.4, ellipsoids=40, seed=3,
@@ -54,16 +57,16 @@ def make_params(cfg, filter_type="shepp-logan"):
     }
 
 
-def traffic_from_profile(name, kernel):
-    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled by the
-    copy-kernel calibration), recorded in profiles/<name>.json; None when absent or when the
-    record is for another kernel."""
+def profile_record(name, kernel):
+    """Counter record of rocprofv3 --pmc runs, profiles/<name>.json: HBM bytes per launch
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration) and texture-unit busy
+    fractions; {} when absent or when the record is for another kernel."""
     path = os.path.join(ROOT, "profiles", name + ".json")
     if not os.path.exists(path):
-        return None
+        return {}
     with open(path) as f:
         rec = json.load(f)
-    return rec.get("traffic_bytes") if rec.get("kernel") == kernel else None
+    return rec if rec.get("kernel") == kernel else {}
 
 
 def cpu_baseline(params, q_gpu, geom, dev, sample_views=(0, 1), slab=8):
@@ -138,8 +141,8 @@ def main(argv=None):
                     help="also run one device copy of this many MiB (rocprof counter calibration)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is designed, not "
-                                   "built (see DESIGN.md)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU; the view-sharded multi-GPU layout is "
+                                   "designed, not built", "n_gpus": args.gpus}))
         return 2
 
     import torch
@@ -215,6 +218,8 @@ def main(argv=None):
     updates = float(nvox) * geom.nview
     fp_s = float(np.median(fp_times))
     ray_slab = float(ncell) * geom.ny  # every ray crosses n_slab slabs (ny == nx here)
+    bp_prof = profile_record(f"dd_backproject_{args.config}_traffic", BP_KERNEL)
+    fp_prof = profile_record(f"dd_forward_{args.config}_traffic", "dd_forward_column_kernel")
     out = {
         "metric": "FDK 512³/720-view recon s; DD proj GUPS; SART vs learned time-to-RMSE",
         "value": sec, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -227,11 +232,13 @@ def main(argv=None):
                    "filter": geom.filter_type, "step": "fdk_filter + dd_backproject (mode ref)"},
         "roofline": {"bound": "hbm", "kernel": BP_KERNEL, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "algorithmic_bytes": bp_bytes,
-                     "traffic": traffic_from_profile(f"dd_backproject_{args.config}_traffic", BP_KERNEL)},
+                     "algorithmic_bytes": bp_bytes, "traffic": bp_prof.get("traffic_bytes"),
+                     "texture_busy": {"ta": bp_prof.get("ta_busy_frac"), "td": bp_prof.get("td_busy_frac")}},
         "fdk_backprojection": {"s": bp_s, "gups": updates / bp_s / 1e9},
         "fdk_filter": {"s": sec - bp_s},
-        "dd_forward": {"s": fp_s, "gups": updates / fp_s / 1e9, "ray_slab_updates_per_s": ray_slab / fp_s},
+        "dd_forward": {"s": fp_s, "gups": updates / fp_s / 1e9, "ray_slab_updates_per_s": ray_slab / fp_s,
+                       "hbm_traffic_bytes": fp_prof.get("traffic_bytes"),
+                       "texture_busy": {"ta": fp_prof.get("ta_busy_frac"), "td": fp_prof.get("td_busy_frac")}},
         "sart_vs_learned_time_to_rmse": None if args.no_learned else learned_vs_sart(),
         "setup_s": t_setup,
     }

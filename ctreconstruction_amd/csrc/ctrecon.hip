@@ -273,9 +273,13 @@ constexpr int kSeg = 4 * kSegLanes;
 // adjacent views of one column group, then the XCD's next column group, so the workgroups
 // resident on an XCD at one time trace nearby rays and share volume lines in its L2.
 #ifndef CTR_FP_VIEW_BLOCK
-#define CTR_FP_VIEW_BLOCK 32
+#define CTR_FP_VIEW_BLOCK 64
 #endif
 constexpr int kViewBlock = CTR_FP_VIEW_BLOCK;
+#ifndef CTR_FP_XCD_RUN
+#define CTR_FP_XCD_RUN 1
+#endif
+constexpr int kXcdRun = CTR_FP_XCD_RUN;  // adjacent column groups per XCD turn
 
 // two consecutive breakpoints of a z-prefix line
 struct PTaps {
@@ -293,17 +297,17 @@ __global__ void __launch_bounds__(256) dd_forward_column_kernel(
     const float* __restrict__ tabs, const float* __restrict__ rows,
     const int32_t* __restrict__ branch, int nx, int ny, int nz, int nzp, int nu, int nv, int stride,
     float zero_tol, int n_views, int col_groups, int n_vblocks) {
-    // workgroup b runs on XCD b % 8, which takes the column groups cg = xcd + 8 c: every XCD
-    // gets an equal share of every band of detector rows (the rows outside the volume's shadow
-    // are cheap).  On one XCD the local index runs over (row block, view block, c, view in
-    // block), fastest last.
+    // workgroup b runs on XCD b % 8, which takes the runs of kXcdRun column groups
+    // cg = (8 c + xcd) kXcdRun + m: every XCD gets an equal share of every band of detector
+    // rows (the rows outside the volume's shadow are cheap).  On one XCD the local index runs
+    // over (row block, view block, c, m, view in block), fastest last.
     const int b = blockIdx.x;
     const int xcd = b & 7, l = b >> 3;
-    const int cg8 = (col_groups + 7) >> 3;
+    const int runs = (col_groups + 8 * kXcdRun - 1) / (8 * kXcdRun);
     const int vi = l % kViewBlock;
     const int r1 = l / kViewBlock;
-    const int cg = (r1 % cg8) * 8 + xcd;
-    const int r2 = r1 / cg8;
+    const int cg = (((r1 / kXcdRun) % runs) * 8 + xcd) * kXcdRun + r1 % kXcdRun;
+    const int r2 = r1 / (kXcdRun * runs);
     const int view = (r2 % n_vblocks) * kViewBlock + vi;
     const int rb = r2 / n_vblocks;
     if (view >= n_views || cg >= col_groups) return;
@@ -1499,7 +1503,8 @@ int ctr_dd_forward(const ctr_geometry* g, const double* angles, const int32_t* b
     const int col_groups = (g->nu + kColWaves - 1) / kColWaves;
     const int row_blocks = (g->nv + kColRows - 1) / kColRows;
     const int n_vblocks = (g->n_views + kViewBlock - 1) / kViewBlock;
-    const long long n_blocks = 8LL * ((col_groups + 7) / 8) * row_blocks * n_vblocks * kViewBlock;
+    const long long n_blocks =
+        8LL * kXcdRun * ((col_groups + 8 * kXcdRun - 1) / (8 * kXcdRun)) * row_blocks * n_vblocks * kViewBlock;
     if (n_blocks >= 2147483647LL) return CTR_ERR_INVALID;
     hipLaunchKernelGGL(dd_forward_column_kernel, dim3((unsigned)n_blocks), dim3(256), 0, s, t_ys, t_xs,
                        proj, tabs, rows, branch, g->nx, g->ny, g->nz, nzp, g->nu, g->nv, stride, 1e-6f,
