@@ -1647,7 +1647,9 @@ int ctr_dd_backproject(const ctr_geometry* g, const double* angles, const int32_
     if (tiles >= 2147483647LL) return CTR_ERR_INVALID;
     const int curved = g->detector_shape == CTR_DETECTOR_CURVED ? 1 : 0;
     const float fdx = (float)g->dx, fdy = (float)(-g->dy), fdz = (float)(-g->dz);
-    if (zero_tol > 0.0f) {  // weight zeroing: the general kernel does every view
+    // weight zeroing, or a volume of one z chunk (a slice through the source height, as in a 2D
+    // fan, is on neither side of it): the general kernel does every view
+    if (zero_tol > 0.0f || tiles_z == 1) {
         hipLaunchKernelGGL(HIP_KERNEL_NAME(dd_backproject_tile_kernel<ZC, kBpAll>), dim3((unsigned)tiles),
                            dim3(256), 0, s, proj, vol, vt, (uint8_t*)nullptr, g->nx, g->ny, g->nz, g->nu,
                            g->nv, g->n_views, x0, y0, z0, fdx, fdy, fdz, curved, scale, zero_tol, accumulate,
