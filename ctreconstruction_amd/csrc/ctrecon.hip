@@ -77,15 +77,16 @@ struct FpLayout {
     size_t tabs_off, rows_off, volt_off, copy_bytes, total;
 };
 
-// entries per line of the z-prefix tables: breakpoints k = 0 .. nz and a copy of the total at
-// nz + 1, so that the pair (k, k+1) around any point of [0, nz] is inside the line
-int zprefix_stride(int nz) { return nz + 2; }
+// entries per line of the z-prefix tables: breakpoints k = 0 .. nz and copies of the total up
+// to a multiple of 4, so that the pair (k, k+1) around any point of [0, nz] is inside the line
+// and every line starts 16-byte aligned
+int zprefix_stride(int nz) { return (nz + 2 + 3) & ~3; }
 
 // workspace: tables, row table, then the z-prefix tables of the volume (k fastest) twice:
 // [y][x][k] (gathered by y-slab views) and [x][y][k] (x-slab views)
 FpLayout fp_layout(const ctr_geometry* g) {
     FpLayout L;
-    // + 1024: a staged segment (up to 168 breakpoints) of a shorter line reads past its end
+    // + 1024: a staged segment (up to 168 breakpoints) may read past the end of the last line
     const size_t vbytes = align_up((size_t)g->nx * g->ny * zprefix_stride(g->nz) * sizeof(float) + 1024, 256);
     L.tabs_off = 0;
     L.rows_off = align_up(L.tabs_off + (size_t)fp_view_stride(g->nu) * g->n_views * 4, 256);
@@ -430,11 +431,13 @@ __global__ void __launch_bounds__(256) dd_forward_column_kernel(
                     // same arithmetic as each lane's (the edges are monotone in the lane)
                     const int kf = (int)floorf(__builtin_amdgcn_fmed3f(fmaf(nu_first, tt, gam), 0.0f, fnz));
                     const int kl = (int)floorf(__builtin_amdgcn_fmed3f(fmaf(nu_last, tt, gam), 0.0f, fnz));
-                    const int kmin = min(kf, kl), span = max(kf, kl) - kmin;
+                    // the segment starts at a multiple of 4 breakpoints (16-byte aligned loads);
+                    // past the end of a line it reads the next one (or the table's slack),
+                    // which no lane uses
+                    const int kseg = min(kf, kl) & ~3, span = max(kf, kl) - kseg;
                     const int nl = (span <= kSeg - 2) ? 1 : (span <= 2 * kSeg - 2) ? 2 : 0;
-                    const int kseg = min(kmin, max(nzp - nl * kSeg, 0));
                     // quads per line that hold breakpoints kseg .. kmax + 1 (<= 21 nl)
-                    const int nq = (max(kf, kl) - kseg + 5) >> 2;
+                    const int nq = (span + 5) >> 2;
                     b_kd = kseg | (min(da, 2) << 16) | (nq << 18);  // kseg <= nz < 2^16 (host check)
                     b_mode = (da > 2 || nl == 0) ? 2 : (nl == 1) ? 1 : 3;
                 }
