@@ -100,14 +100,26 @@ def test_workspace_too_small_is_rejected(lib):
     geom = ScanGeometry(make_params())
     g = geom.ctypes_struct()
     need = lib.ctr_dd_forward_workspace_bytes(ctypes.byref(g))
-    # tables + transposed volume
-    assert need >= 16 * 16 * 16 * 4
+    # tables + two z-prefix tables of the volume
+    assert need >= 2 * 16 * 16 * 16 * 4
     dummy = ctypes.c_void_p(16)
     assert lib.ctr_dd_forward(ctypes.byref(g), dummy, dummy, dummy, dummy, dummy, need - 1, None) == \
         N.CTR_ERR_WORKSPACE
     nbp = lib.ctr_dd_backproject_workspace_bytes(ctypes.byref(g), 0)
     assert lib.ctr_dd_backproject(ctypes.byref(g), dummy, dummy, dummy, dummy, 0, 1.0, 0.0, 0,
                                   dummy, nbp - 1, None) == N.CTR_ERR_WORKSPACE
+
+
+def test_forward_offset_limit_is_rejected_before_launch(lib):
+    # 256 (nz + 2) max(nx, ny) >= 2^31: a batch of 64 slabs would need offsets past 31 bits
+    geom = ScanGeometry(make_params())
+    g = geom.ctypes_struct()
+    g.nx, g.ny, g.nz = 8192, 1, 1024
+    need = lib.ctr_dd_forward_workspace_bytes(ctypes.byref(g))
+    assert need > 0
+    dummy = ctypes.c_void_p(16)  # never dereferenced: the check comes before any launch
+    assert lib.ctr_dd_forward(ctypes.byref(g), dummy, dummy, dummy, dummy, dummy, need, None) == \
+        N.CTR_ERR_UNSUPPORTED
 
 
 def test_bad_filter_arguments_are_rejected(lib):
