@@ -7,7 +7,7 @@
 //   bp_geometry_kernel          per-view backprojection constants, FP64 -> FP32
 //   dd_backproject_tile_kernel  one workgroup per 16 x 16 x ZC voxel tile, windows in LDS
 //   adj_geometry_kernel         per-view constants of the inverse central projection
-//   adj_prescale_kernel         projections times path length over row width
+//   adj_rowprefix_kernel        row prefix sums of projections times path length
 //   dd_adjoint_kernel           exact transpose of the forward projector, gathered per voxel
 //   spectrum_kernel             filter magnitude from the Ram-Lak cosine series, FP64
 //   twiddle_kernel              exp(-2 pi i k / N) table, FP64 -> FP32
@@ -1058,11 +1058,22 @@ __global__ void __launch_bounds__(256) fill_zero_kernel(uint32_t* __restrict__ p
 // forward projector's tables).  The detector columns whose U-edge lines bracket [ka, ka+1] at
 // slab n are found by inverting the central projection through the source (edge coordinate
 // of the ray through a point), with a rounding margin; every candidate then gets the
-// forward projector's own overlap weights from the same FP64-built tables, so a tap carries
-// y * L * o_a * o_c / (w_a * w_c) as the forward sum carries it (transverse overlaps below
-// 1e-6 of the footprint dropped, z overlaps kept, as there).
+// forward projector's own transverse weight o_a / w_a (overlaps below 1e-6 of the footprint
+// dropped, as there).
+//
+// z direction: the forward projector gives row k of column j at this slab
+// L_k o_a / (w_a tt (nu_k+1 - nu_k)) times the z integral of the voxel line over the row's
+// footprint [c_k, c_k+1], c_e = nu_e tt + gamma, with L_k = sec_j sqrt(1 + s2_k^2).  The row
+// edges are equally spaced in nu (nu_e = nuA + nuB e), so a row's footprint is the unit step
+// [k, k+1] in the edge coordinate e(c) = ((c - gamma) / tt - nuA) / nuB, and voxel kc receives
+// sgn(nuB tt) o_a / w_a (Y(e(kc + 1)) - Y(e(kc))) with Y the linear interpolation of the
+// row prefix Y_e = sum_{k < e} y_k L_k (adj_rowprefix_kernel): one table lookup per voxel
+// boundary instead of a walk over rows.
 constexpr int kAdjViewStride = 16;
-constexpr int kAdjZChunk = 8;
+#ifndef CTR_ADJ_ZC
+#define CTR_ADJ_ZC 16
+#endif
+constexpr int kAdjZChunk = CTR_ADJ_ZC;  // voxels along z per work-item
 
 // Per-view constants (FP64 -> FP32):
 //   [0] branch  [1] Pt: source transverse index coordinate  [2] Ps: source slab coordinate
@@ -1120,32 +1131,57 @@ __device__ inline float adj_edge_coord(const float* A, float tk, float n, int cu
     return (u2 - A[9]) * A[10];
 }
 
-// y'[view][k][j] = y[view][k][j] * L(j, k) / |nu_k+1 - nu_k|: the path-length factor and the
-// per-row part of the z normalisation of the forward projector, formed as it forms them.
-__global__ void __launch_bounds__(256) adj_prescale_kernel(const float* __restrict__ y,
-                                                           float* __restrict__ yl,
-                                                           const float* __restrict__ tabs,
-                                                           const float* __restrict__ rows, int nu,
-                                                           int nv, int n_cells, int stride) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n_cells) return;
-    const int j = c % nu;
-    const int r = c / nu;
-    const int k = r % nv, view = r / nv;
+// y[view][k][j] -> Y[view][j][e], e = 0 .. nvp-1: Y_e = sum_{k < e} y_k L_k (L_k = sec_j
+// sqrt(1 + s2_k^2), evaluated as the forward projector evaluates it), summed in FP64, stored in
+// FP32; entries past nv repeat the total.  One workgroup per (view, 32 columns): 32 x 32 tiles
+// through LDS, coalesced along j on read and along e on write; lanes 0..31 carry the sums.
+__global__ void __launch_bounds__(256) adj_rowprefix_kernel(const float* __restrict__ y,
+                                                            float* __restrict__ Y,
+                                                            const float* __restrict__ tabs,
+                                                            const float* __restrict__ rows, int nu,
+                                                            int nv, int nvp, int stride) {
+    __shared__ float tile[32][33];  // [k][j]
+    __shared__ float line[32][33];  // [j][e]
+    const int j0 = blockIdx.x * 32, view = blockIdx.y;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
     const float* T = tabs + (size_t)view * stride;
-    const float sec = T[4 * nu + 2 + j], inv = T[5 * nu + 2 + j];
-    const float s2 = rows[nv + 1 + k] * inv;
-    const float dnu = fabsf(rows[k + 1] - rows[k]);
-    yl[c] = y[c] * (sec * sqrtf(1.0f + s2 * s2) / dnu);
+    const int jj = min(j0 + tx, nu - 1);
+    const float sec = T[4 * nu + 2 + jj], inv = T[5 * nu + 2 + jj];
+    double run = 0.0;  // running sum of column j0 + threadIdx.x (threads 0..31)
+    for (int k0 = 0; k0 < nvp; k0 += 32) {
+        for (int r = ty; r < 32; r += 8) {
+            const int k = k0 + r, j = j0 + tx;
+            float v = 0.0f;
+            if (k < nv && j < nu) {
+                const float s2 = rows[nv + 1 + k] * inv;
+                v = y[((size_t)view * nv + k) * nu + j] * (sec * sqrtf(1.0f + s2 * s2));
+            }
+            tile[r][tx] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x < 32) {
+            for (int r = 0; r < 32; ++r) {
+                line[threadIdx.x][r] = (float)run;
+                run += (double)tile[r][threadIdx.x];
+            }
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < 32 * 32; e += 256) {
+            const int jl = e >> 5, el = e & 31;
+            const int j = j0 + jl, k = k0 + el;
+            if (j < nu && k < nvp) Y[((size_t)view * nu + j) * nvp + k] = line[jl][el];
+        }
+        __syncthreads();
+    }
 }
 
 // One work-item per voxel column chunk of kAdjZChunk voxels.  Per view and candidate detector
 // column the rows that reach the chunk are walked once; each row's z footprint (the forward
 // projector's own fma of the same table values) is overlapped with every voxel of the chunk.
 __global__ void __launch_bounds__(256) dd_adjoint_kernel(
-    const float* __restrict__ yl, float* __restrict__ vol, const float* __restrict__ tabs,
-    const float* __restrict__ rows, const float* __restrict__ at, int nx, int ny, int nz, int nu,
-    int nv, int n_views, int stride, int curved, float scale, float zero_tol, int accumulate) {
+    const float* __restrict__ Y, float* __restrict__ vol, const float* __restrict__ tabs,
+    const float* __restrict__ at, int nx, int ny, int nz, int nu, int nv, int nvp, int n_views,
+    int stride, int curved, float scale, float zero_tol, int accumulate) {
     const int ix = blockIdx.x * blockDim.x + threadIdx.x;
     const int iy = blockIdx.y * blockDim.y + threadIdx.y;
     const int iz0 = blockIdx.z * kAdjZChunk;
@@ -1155,7 +1191,7 @@ __global__ void __launch_bounds__(256) dd_adjoint_kernel(
 #pragma unroll
     for (int i = 0; i < kAdjZChunk; ++i) acc[i] = 0.0f;
     const float tol = zero_tol;
-    const float fz0 = (float)iz0, fz1 = (float)(iz0 + nzc);
+    const float fz0 = (float)iz0, fnv = (float)nv;
     for (int view = 0; view < n_views; ++view) {
         const float* A = at + (size_t)view * kAdjViewStride;
         const int br = (int)A[0];
@@ -1171,7 +1207,8 @@ __global__ void __launch_bounds__(256) dd_adjoint_kernel(
         const int j1 = min((int)floorf(fmaxf(e1, e2) + 0.01f), nu - 1);
         const float* T = tabs + (size_t)view * stride;
         const float gam = T[6 * nu + 2];
-        const float* yv = yl + (size_t)view * nv * nu;
+        const float nuA = A[11], inuB = A[12];
+        const float* Yv = Y + (size_t)view * nu * nvp;
         for (int j = j0; j <= j1; ++j) {
             const float a1 = fmaf(T[nu + 1 + j], fn, T[j]);
             const float a2 = fmaf(T[nu + 2 + j], fn, T[j + 1]);
@@ -1181,33 +1218,24 @@ __global__ void __launch_bounds__(256) dd_adjoint_kernel(
             if (!(oa > 0.0f) || oa < tol * wa) continue;
             const float t = fmaf(T[3 * nu + 2 + j], fn, T[2 * nu + 2 + j]);
             const float rt = __builtin_amdgcn_rcpf(t);
-            // normalisation as the forward projector forms it: the z width |nu_k+1 - nu_k| |t|
-            // is split into |t| (here, with the transverse width) and a per-row factor (in y')
-            const float wnorm = oa * __builtin_amdgcn_rcpf(wa * fabsf(t));
-            // rows whose edges bracket the chunk [iz0, iz0 + nzc]: nu_e = (c - gam) / t,
-            // e = (nu_e - nuA) / nuB
-            const float r1 = ((fz0 - gam) * rt - A[11]) * A[12];
-            const float r2 = ((fz1 - gam) * rt - A[11]) * A[12];
-            const int k0 = max((int)floorf(fminf(r1, r2) - 0.01f), 0);
-            const int k1 = min((int)floorf(fmaxf(r1, r2) + 0.01f), nv - 1);
-            float csum[kAdjZChunk];
+            // edge coordinate of the chunk's voxel boundaries: e_i = e_0 + i de
+            const float de = rt * inuB;
+            const float eb = fmaf(fz0 - gam, de, -nuA * inuB);
+            const float kap = (de > 0.0f ? oa : -oa) * __builtin_amdgcn_rcpf(wa);  // sgn(nuB tt) o_a / w_a
+            const char* col = reinterpret_cast<const char*>(Yv + (size_t)j * nvp);
+            auto lookup = [&](float e) {
+                const float ec = __builtin_amdgcn_fmed3f(e, 0.0f, fnv);
+                const int k = (int)ec;
+                const PTaps p = *reinterpret_cast<const PTaps*>(col + 4 * k);
+                return fmaf(__builtin_amdgcn_fractf(ec), p.b - p.a, p.a);
+            };
+            float prev = lookup(eb);
 #pragma unroll
-            for (int i = 0; i < kAdjZChunk; ++i) csum[i] = 0.0f;
-            for (int k = k0; k <= k1; ++k) {
-                const float c1 = fmaf(rows[k + 1], t, gam);
-                const float c2 = fmaf(rows[k], t, gam);
-                const float clo = fminf(c1, c2), chi = fmaxf(c1, c2);
-                const float yk = yv[(size_t)k * nu + j];
-#pragma unroll
-                for (int i = 0; i < kAdjZChunk; ++i) {
-                    const float fkc = fz0 + (float)i;
-                    const float oc = fminf(chi, fkc + 1.0f) - fmaxf(clo, fkc);
-                    const float w = (oc > 0.0f) ? oc : 0.0f;
-                    csum[i] = fmaf(yk, w, csum[i]);
-                }
+            for (int i = 0; i < kAdjZChunk; ++i) {
+                const float cur = lookup(fmaf((float)(i + 1), de, eb));
+                acc[i] = fmaf(kap, cur - prev, acc[i]);
+                prev = cur;
             }
-#pragma unroll
-            for (int i = 0; i < kAdjZChunk; ++i) acc[i] = fmaf(csum[i], wnorm, acc[i]);
         }
     }
     for (int i = 0; i < nzc; ++i) {
@@ -1567,11 +1595,15 @@ int ctr_fdk_filter(const ctr_geometry* g, const double* spectrum, const float* p
         (float)(g->det_offset[1] * mag), (float)(tau / N)));
 }
 
-// Adjoint workspace: the forward projector's tables and row table (no transposed volume),
-// then the per-view adjoint constants.
+// Adjoint workspace: the forward projector's tables and row table (no volume tables), the
+// per-view adjoint constants, then the row-prefix table Y[view][column][nvp].
 struct AdjLayout {
     size_t tabs_off, rows_off, at_off, yl_off, total;
 };
+
+// entries per column of the adjoint's row-prefix table: breakpoints e = 0 .. nv and a copy
+// of the total, so that the pair (k, k+1) around any point of [0, nv] is inside the column
+int adj_prefix_stride(int nv) { return nv + 2; }
 
 AdjLayout adj_layout(const ctr_geometry* g) {
     AdjLayout L;
@@ -1579,7 +1611,7 @@ AdjLayout adj_layout(const ctr_geometry* g) {
     L.rows_off = align_up((size_t)fp_view_stride(g->nu) * g->n_views * 4, 256);
     L.at_off = align_up(L.rows_off + (size_t)(2 * g->nv + 1) * 4, 256);
     L.yl_off = align_up(L.at_off + (size_t)g->n_views * kAdjViewStride * 4, 256);
-    L.total = align_up(L.yl_off + (size_t)g->n_views * g->nv * g->nu * 4, 256);
+    L.total = align_up(L.yl_off + (size_t)g->n_views * g->nu * adj_prefix_stride(g->nv) * 4, 256);
     return L;
 }
 
@@ -1623,14 +1655,14 @@ int ctr_dd_backproject(const ctr_geometry* g, const double* angles, const int32_
         hipLaunchKernelGGL(adj_geometry_kernel, dim3((g->n_views + 255) / 256), dim3(256), 0, s, *g,
                            angles, branch, at);
         if ((st = status_from_hip(hipGetLastError()))) return st;
-        const int n_cells = g->n_views * g->nv * g->nu;
-        hipLaunchKernelGGL(adj_prescale_kernel, dim3((n_cells + 255) / 256), dim3(256), 0, s, proj, yl,
-                           tabs, rows, g->nu, g->nv, n_cells, stride);
+        const int nvp = adj_prefix_stride(g->nv);
+        hipLaunchKernelGGL(adj_rowprefix_kernel, dim3((g->nu + 31) / 32, g->n_views), dim3(256), 0, s, proj,
+                           yl, tabs, rows, g->nu, g->nv, nvp, stride);
         if ((st = status_from_hip(hipGetLastError()))) return st;
         const dim3 block(64, 4);
         const dim3 grid((g->nx + 63) / 64, (g->ny + 3) / 4, (g->nz + kAdjZChunk - 1) / kAdjZChunk);
-        hipLaunchKernelGGL(dd_adjoint_kernel, grid, block, 0, s, yl, vol, tabs, rows, at, g->nx,
-                           g->ny, g->nz, g->nu, g->nv, g->n_views, stride,
+        hipLaunchKernelGGL(dd_adjoint_kernel, grid, block, 0, s, yl, vol, tabs, at, g->nx, g->ny, g->nz,
+                           g->nu, g->nv, nvp, g->n_views, stride,
                            g->detector_shape == CTR_DETECTOR_CURVED ? 1 : 0, scale, 1e-6f, accumulate);
         return status_from_hip(hipGetLastError());
     }
