@@ -1646,7 +1646,7 @@ int ctr_dd_backproject(const ctr_geometry* g, const double* angles, const int32_
         float* tabs = (float*)(ws + L.tabs_off);
         float* rows = (float*)(ws + L.rows_off);
         float* at = (float*)(ws + L.at_off);
-        float* yl = (float*)(ws + L.yl_off);
+        float* yp = (float*)(ws + L.yl_off);  // row-prefix table
         const int stride = fp_view_stride(g->nu);
         const int gx = (std::max(g->nu + 1, g->nv + 1) + 255) / 256;
         hipLaunchKernelGGL(fp_geometry_kernel, dim3(gx, g->n_views), dim3(256), 0, s, *g, angles,
@@ -1657,11 +1657,11 @@ int ctr_dd_backproject(const ctr_geometry* g, const double* angles, const int32_
         if ((st = status_from_hip(hipGetLastError()))) return st;
         const int nvp = adj_prefix_stride(g->nv);
         hipLaunchKernelGGL(adj_rowprefix_kernel, dim3((g->nu + 31) / 32, g->n_views), dim3(256), 0, s, proj,
-                           yl, tabs, rows, g->nu, g->nv, nvp, stride);
+                           yp, tabs, rows, g->nu, g->nv, nvp, stride);
         if ((st = status_from_hip(hipGetLastError()))) return st;
         const dim3 block(64, 4);
         const dim3 grid((g->nx + 63) / 64, (g->ny + 3) / 4, (g->nz + kAdjZChunk - 1) / kAdjZChunk);
-        hipLaunchKernelGGL(dd_adjoint_kernel, grid, block, 0, s, yl, vol, tabs, at, g->nx, g->ny, g->nz,
+        hipLaunchKernelGGL(dd_adjoint_kernel, grid, block, 0, s, yp, vol, tabs, at, g->nx, g->ny, g->nz,
                            g->nu, g->nv, nvp, g->n_views, stride,
                            g->detector_shape == CTR_DETECTOR_CURVED ? 1 : 0, scale, 1e-6f, accumulate);
         return status_from_hip(hipGetLastError());

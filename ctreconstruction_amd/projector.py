@@ -1,7 +1,7 @@
 """Torch-level operators over the HIP library: DD forward projection, FDK filter, backprojection.
 
 Tensors stay on the GPU; the library enqueues its kernels on torch's current HIP stream and
-returns.  Scratch memory (per-view tables, the x/y-transposed volume, twiddles) is a torch
+returns.  Scratch memory (per-view tables, the z-prefix tables of the volume, twiddles) is a torch
 uint8 tensor handed to the library as its workspace, so the caching allocator owns it.
 Inputs must be float32, contiguous and on a ROCm device: there is no CPU path.
 """

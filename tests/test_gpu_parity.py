@@ -38,7 +38,30 @@ CASES = {
     "curved_fan_2d": dict(NumberOfImage=[48, 48, 1], ImagePixelSpacing=[0.5, 0.5, 0.5],
                           NumberOfDetectorPixels=[96, 1], DetectorPixelSize=[0.5, 0.75], NumberOfViews=24,
                           DetectorShape="Curved"),
+    # a column wavefront's 64 row edges span 83..166 z cells at most slabs: the forward projector
+    # stages its z segment with two loads per slab
+    "tall_rows": dict(NumberOfImage=[20, 20, 200], NumberOfDetectorPixels=[36, 128],
+                      DetectorPixelSize=[1.0, 1.05], NumberOfViews=8),
+    # ... more than 166 cells: the forward projector's explicit per-line path
+    "coarse_rows": dict(NumberOfImage=[12, 12, 256], NumberOfDetectorPixels=[24, 72],
+                        DetectorPixelSize=[1.2, 2.4], NumberOfViews=6),
 }
+
+
+def _edge_span(p, rows=64):
+    """z cells spanned at the isocentre by `rows` consecutive row edges of the geometry, and
+    whether the volume's z extent covers every detector row (no clamping shortens the span)."""
+    mag = (p["SourceInit"][1] - p["DetectorInit"][1]) / p["SourceInit"][1]
+    dv, dz = p["DetectorPixelSize"][1], p["ImagePixelSpacing"][2]
+    covered = p["NumberOfImage"][2] * dz * mag > p["NumberOfDetectorPixels"][1] * dv
+    return (rows - 1) * dv / (mag * dz), covered
+
+
+def test_segment_cases_reach_their_paths():
+    span, covered = _edge_span(make_params(**CASES["tall_rows"]))
+    assert covered and 82 < span < 166
+    span, covered = _edge_span(make_params(**CASES["coarse_rows"]))
+    assert covered and span > 166
 
 
 def _rel(a, b):
