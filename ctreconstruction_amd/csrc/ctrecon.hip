@@ -1309,80 +1309,137 @@ __device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x
 
 constexpr int kFftThreads = 256;
 
-// In-LDS Stockham FFT of length 2^LOG2N (one radix-2 stage when LOG2N is odd, then radix-4
-// stages).  `inv` selects the conjugate twiddles (unnormalised inverse).  Each stage reads
-// all inputs into registers, synchronises, then writes, so one LDS buffer suffices.
+// In-register DFTs of 2, 4, 8 and 16 points (forward: exp(-2 pi i n k / R); `inv` conjugates),
+// results in natural order.  8 = 2 x 4 and 16 = 4 x 4 Cooley-Tukey with the internal twiddles.
+__device__ inline float2 mul_mi(float2 a, bool inv) {  // a * (-i), or a * (+i) for the inverse
+    return inv ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+__device__ inline void dft4(float2& x0, float2& x1, float2& x2, float2& x3, bool inv) {
+    const float2 a0 = cadd(x0, x2), a1 = csub(x0, x2), a2 = cadd(x1, x3);
+    const float2 a3 = mul_mi(csub(x1, x3), inv);
+    x0 = cadd(a0, a2);
+    x1 = cadd(a1, a3);
+    x2 = csub(a0, a2);
+    x3 = csub(a1, a3);
+}
+
+template <int R>
+__device__ inline void dft_reg(float2 (&v)[R], bool inv) {
+    if constexpr (R == 2) {
+        const float2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if constexpr (R == 4) {
+        dft4(v[0], v[1], v[2], v[3], inv);
+    } else if constexpr (R == 8) {
+        // n = 2 n1 + n2 (n1 < 4, n2 < 2), k = k1 + 4 k2
+        constexpr float h = 0.70710678118654752f;
+        dft4(v[0], v[2], v[4], v[6], inv);
+        dft4(v[1], v[3], v[5], v[7], inv);
+        // twiddles W8^(n2 k1) on the odd half: k1 = 1, 2, 3
+        const float s = inv ? 1.0f : -1.0f;
+        v[3] = cmul(v[3], make_float2(h, s * h));
+        v[5] = mul_mi(v[5], inv);
+        v[7] = cmul(v[7], make_float2(-h, s * h));
+        float2 o[8];
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+            o[k1] = cadd(v[2 * k1], v[2 * k1 + 1]);
+            o[k1 + 4] = csub(v[2 * k1], v[2 * k1 + 1]);
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = o[k];
+    } else {
+        static_assert(R == 16, "radix");
+        // n = 4 n1 + n2, k = k1 + 4 k2: DFT4 over n1, twiddle W16^(n2 k1), DFT4 over n2
+#pragma unroll
+        for (int n2 = 0; n2 < 4; ++n2) dft4(v[n2], v[4 + n2], v[8 + n2], v[12 + n2], inv);
+        const float s = inv ? 1.0f : -1.0f;
+        constexpr float c1 = 0.92387953251128674f, s1 = 0.38268343236508978f, h = 0.70710678118654752f;
+        // element [4 k1 + n2] now holds A[n2][k1]
+        v[4 * 1 + 1] = cmul(v[5], make_float2(c1, s * s1));    // W16^1
+        v[4 * 2 + 1] = cmul(v[9], make_float2(h, s * h));      // W16^2
+        v[4 * 3 + 1] = cmul(v[13], make_float2(s1, s * c1));   // W16^3
+        v[4 * 1 + 2] = cmul(v[6], make_float2(h, s * h));      // W16^2
+        v[4 * 2 + 2] = mul_mi(v[10], inv);                     // W16^4
+        v[4 * 3 + 2] = cmul(v[14], make_float2(-h, s * h));    // W16^6
+        v[4 * 1 + 3] = cmul(v[7], make_float2(s1, s * c1));    // W16^3
+        v[4 * 2 + 3] = cmul(v[11], make_float2(-h, s * h));    // W16^6
+        v[4 * 3 + 3] = cmul(v[15], make_float2(-c1, -s * s1)); // W16^9
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) dft4(v[4 * k1], v[4 * k1 + 1], v[4 * k1 + 2], v[4 * k1 + 3], inv);
+        // element [4 k1 + k2] holds X[k1 + 4 k2]
+        float2 o[16];
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1)
+#pragma unroll
+            for (int k2 = 0; k2 < 4; ++k2) o[k1 + 4 * k2] = v[4 * k1 + k2];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) v[k] = o[k];
+    }
+}
+
+// One radix-R Stockham stage over sub-transforms of length ns: butterfly j reads
+// buf[j + r N/R], applies the twiddles exp(-+2 pi i r (j mod ns) / (ns R)), takes an R-point DFT
+// and writes buf[(j / ns) ns R + j mod ns + r ns].  All inputs are read into registers before
+// a barrier, then written, so one LDS buffer suffices.
+template <int R, int N>
+__device__ void stockham_stage(float2* buf, const float2* __restrict__ tw, int ns, bool inv) {
+    constexpr int NBF = N / R;  // butterflies
+    constexpr int NB = (NBF + kFftThreads - 1) / kFftThreads;
+    const int tid = threadIdx.x;
+    const int tstep = N / (ns * R);
+    float2 v[NB][R];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int j = tid + b * kFftThreads;
+        if (j < NBF) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b][r] = buf[j + r * NBF];
+            const int kt = (j % ns) * tstep;
+            if (kt != 0) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    float2 w = tw[r * kt];
+                    if (inv) w.y = -w.y;
+                    v[b][r] = cmul(v[b][r], w);
+                }
+            }
+            dft_reg<R>(v[b], inv);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int j = tid + b * kFftThreads;
+        if (j < NBF) {
+            const int base = (j / ns) * ns * R + (j % ns);
+#pragma unroll
+            for (int r = 0; r < R; ++r) buf[base + r * ns] = v[b][r];
+        }
+    }
+    __syncthreads();
+}
+
+// In-LDS Stockham FFT of length 2^LOG2N: radix-16 stages (two radix-4 levels in registers per
+// LDS round trip), then one radix-2, 4 or 8 stage for the remaining factor.  `inv` selects the
+// conjugate twiddles (unnormalised inverse).
 template <int LOG2N>
 __device__ void fft_lds(float2* buf, const float2* __restrict__ tw, bool inv) {
     constexpr int N = 1 << LOG2N;
-    const int tid = threadIdx.x;
+    constexpr int kRest = LOG2N % 4;
     int ns = 1;
-    if constexpr (LOG2N & 1) {
-        constexpr int NB = (N / 2 + kFftThreads - 1) / kFftThreads;
-        float2 v[NB][2];
+    if constexpr (LOG2N >= 4) {
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            const int j = tid + b * kFftThreads;
-            if (j < N / 2) {
-                v[b][0] = buf[j];
-                v[b][1] = buf[j + N / 2];
-            }
+        for (int st = 0; st < LOG2N / 4; ++st) {
+            stockham_stage<16, N>(buf, tw, ns, inv);
+            ns *= 16;
         }
-        __syncthreads();
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            const int j = tid + b * kFftThreads;
-            if (j < N / 2) {
-                buf[2 * j] = cadd(v[b][0], v[b][1]);
-                buf[2 * j + 1] = csub(v[b][0], v[b][1]);
-            }
-        }
-        __syncthreads();
-        ns = 2;
     }
-    constexpr int NB = (N / 4 + kFftThreads - 1) / kFftThreads;
-    for (; ns < N; ns *= 4) {
-        float2 v[NB][4];
-        const int tstep = N / (ns * 4);
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            const int j = tid + b * kFftThreads;
-            if (j < N / 4) {
-#pragma unroll
-                for (int r = 0; r < 4; ++r) v[b][r] = buf[j + r * (N / 4)];
-                const int kt = (j % ns) * tstep;
-                if (kt != 0) {
-#pragma unroll
-                    for (int r = 1; r < 4; ++r) {
-                        float2 w = tw[r * kt];
-                        if (inv) w.y = -w.y;
-                        v[b][r] = cmul(v[b][r], w);
-                    }
-                }
-                const float2 a0 = cadd(v[b][0], v[b][2]);
-                const float2 a1 = csub(v[b][0], v[b][2]);
-                const float2 a2 = cadd(v[b][1], v[b][3]);
-                const float2 d = csub(v[b][1], v[b][3]);
-                // forward: a3 = -i * d ; inverse: a3 = +i * d
-                const float2 a3 = inv ? make_float2(-d.y, d.x) : make_float2(d.y, -d.x);
-                v[b][0] = cadd(a0, a2);
-                v[b][1] = cadd(a1, a3);
-                v[b][2] = csub(a0, a2);
-                v[b][3] = csub(a1, a3);
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            const int j = tid + b * kFftThreads;
-            if (j < N / 4) {
-                const int base = (j / ns) * ns * 4 + (j % ns);
-#pragma unroll
-                for (int r = 0; r < 4; ++r) buf[base + r * ns] = v[b][r];
-            }
-        }
-        __syncthreads();
-    }
+    if constexpr (kRest == 1) stockham_stage<2, N>(buf, tw, ns, inv);
+    if constexpr (kRest == 2) stockham_stage<4, N>(buf, tw, ns, inv);
+    if constexpr (kRest == 3) stockham_stage<8, N>(buf, tw, ns, inv);
 }
 
 // Filtering() / filter_proj (python/Reconstruction_pycuda.py:223-251): two detector rows per
