@@ -26,6 +26,7 @@ namespace {
 
 typedef __bf16 bf16_t;
 typedef bf16_t bf16x8 __attribute__((ext_vector_type(8)));
+typedef bf16_t bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kConvWaves = 4;    // waves per workgroup
@@ -161,10 +162,11 @@ constexpr int kPPer = (kPHaloChunks + kPThreads - 1) / kPThreads;
 
 __device__ inline int swz(int row, int chunk) { return chunk ^ ((row >> 1) & 7); }
 
+template <bool RELU, bool RESID>
 __global__ void __launch_bounds__(kPThreads) conv3x3_fwd_persistent_kernel(
     const bf16_t* __restrict__ in, const bf16_t* __restrict__ w, const float* __restrict__ bias,
-    const bf16_t* __restrict__ resid, bf16_t* __restrict__ out, int H, int W, int relu,
-    int tiles_x, int tiles_y, int n_tiles) {
+    const bf16_t* __restrict__ resid, bf16_t* __restrict__ out, int H, int W, int tiles_x,
+    int tiles_y, int n_tiles) {
     constexpr int K = 9 * 64;
     __shared__ __attribute__((aligned(16))) bf16_t wl[64 * 9 * 64];
     __shared__ __attribute__((aligned(16))) bf16_t halo[kPHaloPix * 64];
@@ -176,22 +178,32 @@ __global__ void __launch_bounds__(kPThreads) conv3x3_fwd_persistent_kernel(
         *reinterpret_cast<bf16x8*>(&wl[(row * 8 + swz(co, c)) * 8]) =
             *reinterpret_cast<const bf16x8*>(w + (size_t)co * K + (row - co * 9) * 64 + c * 8);
     }
+    // the halo chunks of this thread, the same for every tile: position in the halo relative to
+    // the tile origin (hy, hx >= -1), element offset in the input, element offset in LDS
+    int h_yx[kPPer], h_in[kPPer], h_lds[kPPer];
+#pragma unroll
+    for (int p = 0; p < kPPer; ++p) {
+        const int e = tid + p * kPThreads;
+        const int c = e & 7, pix = min(e >> 3, kPHaloPix - 1);
+        const int hy = pix / kPHaloW - 1, hx = pix % kPHaloW - 1;
+        h_yx[p] = ((hy + 1) << 16) | (hx + 1);
+        h_in[p] = (hy * W + hx) * 64 + c * 8;
+        h_lds[p] = (pix * 8 + swz(pix, c)) * 8;
+    }
     bf16x8 pre[kPPer];
     auto fetch = [&](int tile) {
         const int tx = tile % tiles_x;
         const int ty = (tile / tiles_x) % tiles_y;
         const int n = tile / (tiles_x * tiles_y);
         const int ty0 = ty * kPTileH, tx0 = tx * kPTileW;
+        const bf16_t* base = in + ((size_t)(n * H + ty0) * W + tx0) * 64;
 #pragma unroll
         for (int p = 0; p < kPPer; ++p) {
             const int e = tid + p * kPThreads;
+            const int yy = ty0 - 1 + (h_yx[p] >> 16), xx = tx0 - 1 + (h_yx[p] & 0xffff);
             bf16x8 v = zero8();
-            if (e < kPHaloChunks) {
-                const int c = e & 7, pix = e >> 3;
-                const int yy = ty0 + pix / kPHaloW - 1, xx = tx0 + pix % kPHaloW - 1;
-                if (yy >= 0 && yy < H && xx >= 0 && xx < W)
-                    v = *reinterpret_cast<const bf16x8*>(in + ((size_t)(n * H + yy) * W + xx) * 64 + c * 8);
-            }
+            if (e < kPHaloChunks && (unsigned)yy < (unsigned)H && (unsigned)xx < (unsigned)W)
+                v = *reinterpret_cast<const bf16x8*>(base + h_in[p]);
             pre[p] = v;
         }
     };
@@ -199,17 +211,18 @@ __global__ void __launch_bounds__(kPThreads) conv3x3_fwd_persistent_kernel(
 #pragma unroll
         for (int p = 0; p < kPPer; ++p) {
             const int e = tid + p * kPThreads;
-            if (e < kPHaloChunks) {
-                const int c = e & 7, pix = e >> 3;
-                *reinterpret_cast<bf16x8*>(&halo[(pix * 8 + swz(pix, c)) * 8]) = pre[p];
-            }
+            if (e < kPHaloChunks) *reinterpret_cast<bf16x8*>(&halo[h_lds[p]]) = pre[p];
         }
     };
     const int lane = tid & 63, wave = tid >> 6;
     const int m = lane & 15, g = lane >> 4;
-    float bias_t[4];
+    // the MFMAs take the weights as the row operand, so lane (m, g) holds output channels
+    // 16 t + 4 g + r of pixel m: the bias of those channels starts every accumulator
+    float bias_t[4][4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) bias_t[t] = bias ? bias[t * 16 + m] : 0.0f;
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) bias_t[t][r] = bias ? bias[t * 16 + 4 * g + r] : 0.0f;
     int tile = blockIdx.x;
     if (tile < n_tiles) {
         fetch(tile);
@@ -222,7 +235,7 @@ __global__ void __launch_bounds__(kPThreads) conv3x3_fwd_persistent_kernel(
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
-            for (int t = 0; t < 4; ++t) acc[i][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+            for (int t = 0; t < 4; ++t) acc[i][t] = f32x4{bias_t[t][0], bias_t[t][1], bias_t[t][2], bias_t[t][3]};
         // the previous tile's stores have completed (their data registers are reused by the
         // accumulators); the next tile's halo is then in flight while this tile is computed
         __builtin_amdgcn_s_waitcnt(0);
@@ -248,26 +261,23 @@ __global__ void __launch_bounds__(kPThreads) conv3x3_fwd_persistent_kernel(
                 for (int i = 0; i < 4; ++i)
 #pragma unroll
                     for (int t = 0; t < 4; ++t)
-                        acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[t], acc[i][t], 0, 0, 0);
+                        acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[t], a[i], acc[i][t], 0, 0, 0);
             }
         }
         __syncthreads();  // every wave is done with this halo: reuse it to stage the results
-        // bias + ReLU, as bfloat16, into stage[pixel][channel] (pixel = row * 32 + column, rows
-        // of 8 16-byte chunks swizzled like the halo)
+        // (ReLU), as bfloat16, into stage[pixel][channel] (pixel = row * 32 + column, rows of 8
+        // 16-byte chunks swizzled like the halo): four channels per 8-byte store
         bf16_t* stage = halo;
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int co = t * 16 + m;
-            const float bc = bias_t[t];
+        for (int i = 0; i < 4; ++i) {
+            const int p = (2 * wave + (i >> 1)) * kPTileW + 16 * (i & 1) + m;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
+            for (int t = 0; t < 4; ++t) {
+                const int co = t * 16 + 4 * g;
+                bf16x4 q;
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int p = (2 * wave + (i >> 1)) * kPTileW + 16 * (i & 1) + 4 * g + r;
-                    float v = acc[i][t][r] + bc;
-                    if (relu) v = fmaxf(v, 0.0f);
-                    stage[(p * 8 + swz(p, co >> 3)) * 8 + (co & 7)] = (bf16_t)v;
-                }
+                for (int r = 0; r < 4; ++r) q[r] = (bf16_t)(RELU ? fmaxf(acc[i][t][r], 0.0f) : acc[i][t][r]);
+                *reinterpret_cast<bf16x4*>(&stage[(p * 8 + swz(p, co >> 3)) * 8 + (co & 7)]) = q;
             }
         }
         __syncthreads();
@@ -290,14 +300,14 @@ __global__ void __launch_bounds__(kPThreads) conv3x3_fwd_persistent_kernel(
                     const int yy = ty * kPTileH + p / kPTileW, xx = tx * kPTileW + p % kPTileW;
                     ok[k] = yy < H && xx < W;
                     o[k] = ((size_t)(n * H + min(yy, H - 1)) * W + min(xx, W - 1)) * 64 + c * 8;
-                    rv[k] = (resid && ok[k]) ? *reinterpret_cast<const bf16x8*>(resid + o[k]) : zero8();
+                    if constexpr (RESID) rv[k] = *reinterpret_cast<const bf16x8*>(resid + o[k]);
                 }
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     const int e = tid + (h0 + k) * kPThreads;
                     const int c = e & 7, p = e >> 3;
                     bf16x8 v = *reinterpret_cast<const bf16x8*>(&stage[(p * 8 + swz(p, c)) * 8]);
-                    if (resid) {
+                    if constexpr (RESID) {
 #pragma unroll
                         for (int q = 0; q < 8; ++q) v[q] = (bf16_t)((float)v[q] + (float)rv[k][q]);
                     }
@@ -434,8 +444,14 @@ int ctr_conv3x3_bf16_forward(const void* in, const void* weight, const float* bi
             hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
             return CTR_ERR_HIP;
         const int grid = (int)std::min<long long>(n_tiles, std::max(n_cu, 1));
-        hipLaunchKernelGGL(conv3x3_fwd_persistent_kernel, dim3(grid), dim3(kPThreads), 0, s, x, wt, bias, r,
-                           y, h, w, relu, tiles_x, tiles_y, (int)n_tiles);
+        auto launch = [&](auto kern) {
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(kPThreads), 0, s, x, wt, bias, r, y, h, w, tiles_x,
+                               tiles_y, (int)n_tiles);
+        };
+        if (relu)
+            r ? launch(conv3x3_fwd_persistent_kernel<true, true>) : launch(conv3x3_fwd_persistent_kernel<true, false>);
+        else
+            r ? launch(conv3x3_fwd_persistent_kernel<false, true>) : launch(conv3x3_fwd_persistent_kernel<false, false>);
     } else if (cin % 32 == 0) {
         hipLaunchKernelGGL(HIP_KERNEL_NAME(conv3x3_fwd_kernel<true>), dim3((unsigned)blocks), dim3(256),
                            0, s, x, wt, bias, r, y, n, h, w, cin, cout, relu);
