@@ -16,7 +16,8 @@ PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 REPO_DIR = os.path.dirname(PKG_DIR)
 LIB_DIR = os.path.join(PKG_DIR, "lib")
 LIB_PATH = os.path.join(LIB_DIR, "libctrecon.so")
-SOURCES = [os.path.join(PKG_DIR, "csrc", "ctrecon.hip"), os.path.join(PKG_DIR, "csrc", "prior.hip")]
+SOURCES = [os.path.join(PKG_DIR, "csrc", "ctrecon.hip"), os.path.join(PKG_DIR, "csrc", "prior.hip"),
+           os.path.join(PKG_DIR, "csrc", "pixel.hip")]
 INCLUDE_DIR = os.path.join(REPO_DIR, "include")
 
 # Exported symbols (every function declared in include/ctrecon.h).
@@ -25,6 +26,8 @@ EXPORTS = (
     "ctr_dd_forward_workspace_bytes", "ctr_dd_forward",
     "ctr_fdk_filter_workspace_bytes", "ctr_filter_spectrum", "ctr_fdk_filter",
     "ctr_dd_backproject_workspace_bytes", "ctr_dd_backproject",
+    "ctr_pixel_filter_spectrum", "ctr_pixel_filter",
+    "ctr_pixel_backproject_workspace_bytes", "ctr_pixel_backproject",
     "ctr_conv3x3_bf16_forward", "ctr_conv3x3_bf16_wgrad_workspace_bytes", "ctr_conv3x3_bf16_wgrad",
 )
 
@@ -109,6 +112,10 @@ def _declare(lib):
         "ctr_dd_backproject_workspace_bytes": (c_size, [gp, c_int]),
         "ctr_dd_backproject": (c_int, [gp, c_vp, c_vp, c_vp, c_vp, c_int, c_flt, c_flt, c_int,
                                        c_vp, c_size, c_vp]),
+        "ctr_pixel_filter_spectrum": (c_int, [c_int, c_dbl, c_int, c_dbl, c_vp, c_vp]),
+        "ctr_pixel_filter": (c_int, [gp, c_vp, c_vp, c_vp, c_int, c_vp, c_size, c_vp]),
+        "ctr_pixel_backproject_workspace_bytes": (c_size, [gp]),
+        "ctr_pixel_backproject": (c_int, [gp, c_vp, c_vp, c_vp, c_flt, c_int, c_vp, c_size, c_vp]),
         "ctr_conv3x3_bf16_forward": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_int, c_int, c_int,
                                              c_int, c_int, c_vp]),
         "ctr_conv3x3_bf16_wgrad_workspace_bytes": (c_size, [c_int, c_int, c_int, c_int, c_int]),

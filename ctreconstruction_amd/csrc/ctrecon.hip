@@ -1261,8 +1261,11 @@ int log2_exact(int n) {
 // order, FP64.  The reference takes |fftshift(fft(h))| of the discrete Ram-Lak taps; h is
 // even, so |DFT|(w) = |h0 + 2 sum_{odd n} h(n) cos(n w)| with w = 2 pi f / N (signed f).
 // The cosine table lives in LDS; arguments are reduced exactly as (f*n mod N).
+// ConeBeam.Filter (python/ConeBeamGPU.py:355-384) is the same series times gain = 2 with the
+// mask |w| > mask_w = pi*cutoff; Reconstruction.Filter has gain 1, mask_w = pi*cutoff/(2 tau).
 __global__ void __launch_bounds__(256) spectrum_kernel(double* __restrict__ out, int N,
-                                                       double tau, int ftype, double cutoff) {
+                                                       double tau, int ftype, double cutoff,
+                                                       double gain, double mask_w) {
     extern __shared__ double costab[];
     for (int m = threadIdx.x; m < N; m += blockDim.x) costab[m] = cos(2.0 * kPi * m / N);
     __syncthreads();
@@ -1276,7 +1279,7 @@ __global__ void __launch_bounds__(256) spectrum_kernel(double* __restrict__ out,
         const double hn = -0.5 / ((kPi * tau * n) * (kPi * tau * n));
         acc += hn * costab[(int)(((long long)fm * n) % N)];
     }
-    double H = fabs(h0 + 2.0 * acc);
+    double H = fabs(h0 + 2.0 * acc) * gain;
     const double w = 2.0 * kPi * fs / N;
     switch (ftype) {
         case CTR_FILTER_SHEPP_LOGAN: {
@@ -1289,7 +1292,7 @@ __global__ void __launch_bounds__(256) spectrum_kernel(double* __restrict__ out,
         case CTR_FILTER_HANN: H *= 0.5 + 0.5 * cos(w / cutoff); break;
         default: break;
     }
-    if (fabs(w) > kPi * cutoff / (2.0 * tau)) H = 0.0;
+    if (fabs(w) > mask_w) H = 0.0;
     out[f] = H;
 }
 
@@ -1447,11 +1450,14 @@ __device__ void fft_lds(float2* buf, const float2* __restrict__ tw, bool inv) {
 // ifft(H * fft(a + i b)) = (a*g) + i (b*g) and the two rows separate without extra work.
 // The cosine pre-weight R/sqrt(R^2 + kk^2 + pp^2) is evaluated at the reference's cell-edge
 // coordinates kk = ((j - nu/2) du + off_u) R/SDD, pp = ((nv/2 - k) dv + off_v) R/SDD.
+// ConeBeamGPU's filtering (python/ConeBeamGPU.py:296-302) weighs at cell centres (off_u =
+// du R/(2 SDD), off_v = -dv R/(2 SDD)) and filters fliplr(row): the spectrum is real and
+// even, so that equals the mirror image of the filtered row, written with mirror = 1.
 template <int LOG2N>
 __global__ void __launch_bounds__(kFftThreads) fdk_filter_kernel(
     const float* __restrict__ in, float* __restrict__ out, const double* __restrict__ spec,
     const float2* __restrict__ tw, int n_rows, int nu, int nv, float R, float su, float ou,
-    float sv, float ov, float out_scale) {
+    float sv, float ov, float out_scale, int mirror) {
     constexpr int N = 1 << LOG2N;
     __shared__ float2 buf[N];
     const int rowA = 2 * blockIdx.x;
@@ -1482,23 +1488,24 @@ __global__ void __launch_bounds__(kFftThreads) fdk_filter_kernel(
     fft_lds<LOG2N>(buf, tw, true);
     for (int n = threadIdx.x; n < nu; n += kFftThreads) {
         const float2 val = buf[n];
-        out[(size_t)rowA * nu + n] = val.x * out_scale;
-        if (hasB) out[(size_t)rowB * nu + n] = val.y * out_scale;
+        const int j = mirror ? nu - 1 - n : n;
+        out[(size_t)rowA * nu + j] = val.x * out_scale;
+        if (hasB) out[(size_t)rowB * nu + j] = val.y * out_scale;
     }
 }
 
 template <int L>
 hipError_t launch_filter(int l2, dim3 grid, hipStream_t st, const float* in, float* out,
                          const double* spec, const float2* tw, int n_rows, int nu, int nv, float R,
-                         float su, float ou, float sv, float ov, float scale) {
+                         float su, float ou, float sv, float ov, float scale, int mirror) {
     if (l2 == L) {
         hipLaunchKernelGGL(HIP_KERNEL_NAME(fdk_filter_kernel<L>), grid, dim3(kFftThreads), 0, st,
-                           in, out, spec, tw, n_rows, nu, nv, R, su, ou, sv, ov, scale);
+                           in, out, spec, tw, n_rows, nu, nv, R, su, ou, sv, ov, scale, mirror);
         return hipGetLastError();
     }
     if constexpr (L < kMaxLog2N) {
         return launch_filter<L + 1>(l2, grid, st, in, out, spec, tw, n_rows, nu, nv, R, su, ou, sv,
-                                    ov, scale);
+                                    ov, scale, mirror);
     }
     return hipErrorInvalidValue;
 }
@@ -1620,7 +1627,7 @@ int ctr_filter_spectrum(const ctr_geometry* g, int filter_type, double cutoff, d
     const double tau = g->du * g->sad / g->sdd;
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(spectrum_kernel, dim3((N + 255) / 256), dim3(256), N * sizeof(double), s,
-                       spectrum, N, tau, filter_type, cutoff);
+                       spectrum, N, tau, filter_type, cutoff, 1.0, kPi * cutoff / (2.0 * tau));
     return status_from_hip(hipGetLastError());
 }
 
@@ -1649,7 +1656,52 @@ int ctr_fdk_filter(const ctr_geometry* g, const double* spectrum, const float* p
     return status_from_hip(launch_filter<3>(
         l2, grid, s, proj_in, proj_out, spectrum, tw, n_rows, g->nu, g->nv, (float)R,
         (float)(g->du * mag), (float)(g->det_offset[0] * mag), (float)(g->dv * mag),
-        (float)(g->det_offset[1] * mag), (float)(tau / N)));
+        (float)(g->det_offset[1] * mag), (float)(tau / N), 0));
+}
+
+int ctr_pixel_filter_spectrum(int n_fft, double tau, int filter_type, double cutoff,
+                              double* spectrum, void* stream) {
+    if (!spectrum || !(tau > 0)) return CTR_ERR_INVALID;
+    if (filter_type < CTR_FILTER_RAM_LAK || filter_type > CTR_FILTER_HANN) return CTR_ERR_INVALID;
+    if (filter_type != CTR_FILTER_RAM_LAK && !(cutoff > 0)) return CTR_ERR_INVALID;
+    if (!(cutoff >= 0)) return CTR_ERR_INVALID;
+    const int l2 = log2_exact(n_fft);
+    if (l2 < 1) return CTR_ERR_INVALID;
+    if (l2 > kMaxLog2N) return CTR_ERR_UNSUPPORTED;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(spectrum_kernel, dim3((n_fft + 255) / 256), dim3(256),
+                       n_fft * sizeof(double), s, spectrum, n_fft, tau, filter_type, cutoff, 2.0,
+                       kPi * cutoff);
+    return status_from_hip(hipGetLastError());
+}
+
+int ctr_pixel_filter(const ctr_geometry* g, const double* spectrum, const float* proj_in,
+                     float* q_out, int n_views, void* workspace, size_t workspace_bytes,
+                     void* stream) {
+    int st = check_geometry(g);
+    if (st) return st;
+    if (g->detector_shape != CTR_DETECTOR_FLAT) return CTR_ERR_UNSUPPORTED;
+    if (!spectrum || !proj_in || !q_out || !workspace || n_views <= 0) return CTR_ERR_INVALID;
+    if (proj_in == q_out) return CTR_ERR_INVALID;  // rows are written mirrored
+    if ((double)n_views * g->nv * g->nu >= 2147483647.0) return CTR_ERR_INVALID;
+    const int N = ctr_zero_padded_length(g->nu);
+    if (N <= 0) return CTR_ERR_INVALID;
+    const int l2 = log2_exact(N);
+    if (l2 < 3) return CTR_ERR_UNSUPPORTED;  // nu >= 5
+    if (l2 > kMaxLog2N) return CTR_ERR_UNSUPPORTED;
+    if (workspace_bytes < ctr_fdk_filter_workspace_bytes(g)) return CTR_ERR_WORKSPACE;
+    hipStream_t s = (hipStream_t)stream;
+    float2* tw = (float2*)workspace;
+    hipLaunchKernelGGL(twiddle_kernel, dim3((N + 255) / 256), dim3(256), 0, s, tw, N);
+    if ((st = status_from_hip(hipGetLastError()))) return st;
+    const int n_rows = n_views * g->nv;
+    const double mag = g->sad / g->sdd;
+    const double su = g->du * mag, sv = g->dv * mag;
+    const dim3 grid((n_rows + 1) / 2);
+    return status_from_hip(launch_filter<3>(l2, grid, s, proj_in, q_out, spectrum, tw, n_rows,
+                                            g->nu, g->nv, (float)g->sad, (float)su,
+                                            (float)(0.5 * su), (float)sv, (float)(-0.5 * sv),
+                                            (float)(1.0 / N), 1));
 }
 
 // Adjoint workspace: the forward projector's tables and row table (no volume tables), the

@@ -1,0 +1,340 @@
+// pixel.hip -- pixel-driven (voxel-driven, bilinear) cone-beam backprojection for gfx950.
+//
+// Replaces the per-view kernels CalculateInterpolGrid + Interpol2dLineargpu and the host loop
+// of ConeBeam.Reconstruction (python/ConeBeamGPU.py:65-129, 303-315).  For a voxel at
+// (x, y, z) and a view at angle th: t = x cos + y sin, s = -x sin + y cos,
+// U = R t/(R - s), V = R z/(R - s), W = R^2/(R - s)^2 (R = source-to-axis distance), and the
+// voxel gains scale * W * bilinear(Q_view; U, V) with the bilinear sample 0 outside the
+// detector grid (ki, p) = ((j - (nu-1)/2), (k - (nv-1)/2)) * (du, dv) * R/SDD.
+//
+// Kernels:
+//   px_view_kernel               cos/sin per view, FP64 -> FP32
+//   px_tile_kernel               per (view, 16x16 column tile): range of the detector column
+//                                coordinate and of the magnification R/(R-s) over the tile corners
+//   pixel_backproject_kernel     one workgroup per 16 x 16 x kPxTz voxel tile, one work-item per
+//                                (x, y) column holding kPxTz accumulators; per view the tile's
+//                                detector window is staged in LDS and every voxel reads its four
+//                                samples there (two ds_read2); windows wider than the LDS tile fall
+//                                back to direct reads from global memory for that view.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "ctrecon.h"
+
+namespace {
+
+constexpr int kPxTile = 16;     // in-plane tile edge (voxels)
+constexpr int kPxTz = 32;       // z voxels per work-item
+constexpr int kPxPitch = 65;    // LDS window row pitch (floats); a window has <= 64 columns
+constexpr int kPxRows = 96;     // LDS window rows
+constexpr int kPxThreads = kPxTile * kPxTile;
+
+size_t align_up(size_t n, size_t a) { return (n + a - 1) / a * a; }
+
+int status_from_hip(hipError_t e) { return e == hipSuccess ? CTR_OK : CTR_ERR_HIP; }
+
+struct PxArgs {
+    int nx, ny, nz, nu, nv, n_views;
+    int tiles_x, tiles_y, n_tiles_xy, accumulate;
+    float x_first, y_first, z_first, dx, dy, dz;
+    float R, inv_dk, u_mid, u_last, inv_dp, v_mid, v_last, scale;
+};
+
+struct PxTileArgs {
+    int n_views, tiles_x, n_tiles_xy, nx, ny, pad;
+    double x_first, y_first, dx, dy, R, inv_dk, u_mid;
+};
+
+__global__ void px_view_kernel(const double* __restrict__ angles, float2* __restrict__ cs,
+                               int n_views) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n_views) return;
+    double s, c;
+    sincos(angles[v], &s, &c);
+    cs[v] = make_float2((float)c, (float)s);
+}
+
+// U and R/(R-s) are monotone along every line of the (x, y) plane while R - s > 0, so their
+// extremes over a tile are at its corner voxel centres.  A tile that overhangs the grid is cut
+// at the last voxel: the work-items past it sample at that voxel's position.
+__global__ void px_tile_kernel(const double* __restrict__ angles, float4* __restrict__ tb,
+                               PxTileArgs a) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= a.n_views * a.n_tiles_xy) return;
+    const int v = idx / a.n_tiles_xy;
+    const int t = idx - v * a.n_tiles_xy;
+    const int bx = t % a.tiles_x, by = t / a.tiles_x;
+    double s, c;
+    sincos(angles[v], &s, &c);
+    const double xs[2] = {a.x_first + (double)(bx * kPxTile) * a.dx,
+                          a.x_first + (double)min(bx * kPxTile + kPxTile - 1, a.nx - 1) * a.dx};
+    const double ys[2] = {a.y_first + (double)(by * kPxTile) * a.dy,
+                          a.y_first + (double)min(by * kPxTile + kPxTile - 1, a.ny - 1) * a.dy};
+    double fu_lo = INFINITY, fu_hi = -INFINITY, m_lo = INFINITY, m_hi = -INFINITY;
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) {
+            const double tt = xs[i] * c + ys[j] * s;
+            const double ss = -xs[i] * s + ys[j] * c;
+            const double m = a.R / (a.R - ss);
+            const double fu = tt * m * a.inv_dk + a.u_mid;
+            fu_lo = fmin(fu_lo, fu);
+            fu_hi = fmax(fu_hi, fu);
+            m_lo = fmin(m_lo, m);
+            m_hi = fmax(m_hi, m);
+        }
+    tb[idx] = make_float4((float)fu_lo, (float)fu_hi, (float)m_lo, (float)m_hi);
+}
+
+__global__ void __launch_bounds__(kPxThreads) pixel_backproject_kernel(
+    const float* __restrict__ q, float* __restrict__ vol, const float2* __restrict__ cs,
+    const float4* __restrict__ tb, PxArgs a) {
+    __shared__ float win[kPxRows * kPxPitch];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int tx = tid & (kPxTile - 1), ty = tid / kPxTile;
+    int b = blockIdx.x;
+    const int bx = b % a.tiles_x;
+    b /= a.tiles_x;
+    const int by = b % a.tiles_y;
+    const int bz = b / a.tiles_y;
+    const int tile = by * a.tiles_x + bx;
+    const int ix = bx * kPxTile + tx, iy = by * kPxTile + ty, iz0 = bz * kPxTz;
+    // past the grid's x/y end a work-item samples at the last voxel (its sums are not stored);
+    // past its z end the nominal positions are kept, which only lengthens the window in v
+    const float x = fmaf((float)min(ix, a.nx - 1), a.dx, a.x_first);
+    const float y = fmaf((float)min(iy, a.ny - 1), a.dy, a.y_first);
+    const float zlo = fmaf((float)iz0, a.dz, a.z_first);
+    const float zhi = fmaf((float)(iz0 + kPxTz - 1), a.dz, a.z_first);
+    const size_t view_cells = (size_t)a.nu * a.nv;
+
+    float acc[kPxTz];
+#pragma unroll
+    for (int k = 0; k < kPxTz; ++k) acc[k] = 0.0f;
+
+    for (int v = 0; v < a.n_views; ++v) {
+        const float2 sc = cs[v];
+        const float4 bnd = tb[(size_t)v * a.n_tiles_xy + tile];
+        // window: columns floor(fu_lo)-1 .. floor(fu_hi)+2, rows likewise from the V extremes
+        const int c0 = (int)floorf(bnd.x) - 1;
+        const int cols = (int)floorf(bnd.y) + 3 - c0;
+        const float va = bnd.z * zlo, vb = bnd.w * zlo, vc = bnd.z * zhi, vd = bnd.w * zhi;
+        const float fv_lo = fmaf(fminf(fminf(va, vb), fminf(vc, vd)), a.inv_dp, a.v_mid);
+        const float fv_hi = fmaf(fmaxf(fmaxf(va, vb), fmaxf(vc, vd)), a.inv_dp, a.v_mid);
+        const int r0 = (int)floorf(fv_lo) - 1;
+        const int rows = (int)floorf(fv_hi) + 3 - r0;
+        const bool fits = cols <= kPxPitch - 1 && rows <= kPxRows;
+
+        // this column's sample geometry (python/ConeBeamGPU.py:125-127)
+        const float t = fmaf(x, sc.x, y * sc.y);
+        const float s = fmaf(y, sc.x, -x * sc.y);
+        const float m = a.R / (a.R - s);
+        const float fu = fmaf(t * m, a.inv_dk, a.u_mid);
+        const bool uok = fu >= 0.0f && fu <= a.u_last;
+        const float w = m * m * a.scale;
+        float kz = m * a.dz * a.inv_dp;
+        float fvb = fmaf(m * zlo, a.inv_dp, a.v_mid);
+
+        if (fits) {
+            const float* qv = q + (size_t)v * view_cells;
+            __syncthreads();  // the previous view's window is no longer read
+            float stage[kPxRows / 4];
+#pragma unroll
+            for (int i = 0; i < kPxRows / 4; ++i) {
+                const int r = i * 4 + wave;
+                const int gr = r0 + r, gc = c0 + lane;
+                float val = 0.0f;
+                if (r < rows && lane < cols && (unsigned)gr < (unsigned)a.nv &&
+                    (unsigned)gc < (unsigned)a.nu)
+                    val = qv[(size_t)gr * a.nu + gc];
+                stage[i] = val;
+            }
+#pragma unroll
+            for (int i = 0; i < kPxRows / 4; ++i) {
+                const int r = i * 4 + wave;
+                if (r < rows && lane < cols) win[r * kPxPitch + lane] = stage[i];
+            }
+            __syncthreads();
+
+            // iu = floor(fu) except on the last grid column (fu = nu-1: iu = nu-2, fx = 1)
+            const int iu = min((int)floorf(fu), a.nu - 2);
+            const float fx = fu - (float)iu;
+            const int cu = min(max(iu - c0, 0), cols - 2);
+            // every row index floor(fv(k)) must lie in [r0, r0 + rows - 2]; fv is monotone in k
+            const float f_first = fvb;
+            const float f_last = fmaf((float)(kPxTz - 1), kz, fvb);
+            const bool in_win = floorf(fminf(f_first, f_last)) >= (float)r0 &&
+                                floorf(fmaxf(f_first, f_last)) <= (float)(r0 + rows - 2);
+            const bool ok_col = uok && in_win;
+            if (!in_win) {
+                fvb = (float)r0;
+                kz = 0.0f;
+            }
+            const int colterm = cu - r0 * kPxPitch;
+#pragma unroll
+            for (int k = 0; k < kPxTz; ++k) {
+                const float fv = fmaf((float)k, kz, fvb);
+                const float fl = floorf(fv);
+                const float fy = fv - fl;
+                const int addr = (int)fl * kPxPitch + colterm;
+                const float q00 = win[addr], q01 = win[addr + 1];
+                const float q10 = win[addr + kPxPitch], q11 = win[addr + kPxPitch + 1];
+                const float m0 = fmaf(fx, q01 - q00, q00);
+                const float m1 = fmaf(fx, q11 - q10, q10);
+                const float val = fmaf(fy, m1 - m0, m0);
+                const bool ok = ok_col && fv >= 0.0f && fv <= a.v_last;
+                acc[k] = ok ? fmaf(val, w, acc[k]) : acc[k];
+            }
+        } else {
+            // window wider than the LDS tile: the four samples come from global memory
+            const float* qv = q + (size_t)v * view_cells;
+            const int iu = min(max((int)floorf(fu), 0), a.nu - 2);
+            const float fx = fu - (float)iu;
+#pragma unroll 4
+            for (int k = 0; k < kPxTz; ++k) {
+                const float fv = fmaf((float)k, kz, fvb);
+                const bool ok = uok && fv >= 0.0f && fv <= a.v_last;
+                const int iv = ok ? min((int)floorf(fv), a.nv - 2) : 0;
+                const float fy = fv - (float)iv;
+                const float* r0p = qv + (size_t)iv * a.nu + iu;
+                const float q00 = r0p[0], q01 = r0p[1], q10 = r0p[a.nu], q11 = r0p[a.nu + 1];
+                const float m0 = fmaf(fx, q01 - q00, q00);
+                const float m1 = fmaf(fx, q11 - q10, q10);
+                const float val = fmaf(fy, m1 - m0, m0);
+                acc[k] = ok ? fmaf(val, w, acc[k]) : acc[k];
+            }
+        }
+    }
+
+    if (ix < a.nx && iy < a.ny) {
+#pragma unroll
+        for (int k = 0; k < kPxTz; ++k) {
+            const int iz = iz0 + k;
+            if (iz < a.nz) {
+                const size_t o = ((size_t)iz * a.ny + iy) * a.nx + ix;
+                vol[o] = a.accumulate ? vol[o] + acc[k] : acc[k];
+            }
+        }
+    }
+}
+
+struct PxLayout {
+    size_t cs_off, tb_off, total;
+    int tiles_x, tiles_y, slabs;
+};
+
+PxLayout px_layout(const ctr_geometry* g) {
+    PxLayout l;
+    l.tiles_x = (g->nx + kPxTile - 1) / kPxTile;
+    l.tiles_y = (g->ny + kPxTile - 1) / kPxTile;
+    l.slabs = (g->nz + kPxTz - 1) / kPxTz;
+    l.cs_off = 0;
+    l.tb_off = align_up((size_t)g->n_views * sizeof(float2), 256);
+    l.total = l.tb_off + align_up((size_t)g->n_views * l.tiles_x * l.tiles_y * sizeof(float4), 256);
+    return l;
+}
+
+// Checks of the pixel-driven path: the generic limits of ctr_geometry plus what the
+// reference's ConeBeam geometry has (flat detector, no offset, no helix, nu, nv >= 2 so that
+// ki[1] and p[1] exist) and a source outside every voxel column of the tile grid.
+int check_pixel_geometry(const ctr_geometry* g) {
+    if (!g) return CTR_ERR_INVALID;
+    if (g->nx <= 0 || g->ny <= 0 || g->nz <= 0 || g->n_views <= 0) return CTR_ERR_INVALID;
+    if (g->nu < 2 || g->nv < 2) return CTR_ERR_INVALID;
+    if (!(g->dx > 0) || !(g->dy > 0) || !(g->dz > 0) || !(g->du > 0) || !(g->dv > 0))
+        return CTR_ERR_INVALID;
+    if (!(g->sad > 0) || !(g->sdd > 0)) return CTR_ERR_INVALID;
+    if (g->detector_shape != CTR_DETECTOR_FLAT) return CTR_ERR_UNSUPPORTED;
+    if (g->det_offset[0] != 0.0 || g->det_offset[1] != 0.0 || g->helical != 0.0)
+        return CTR_ERR_UNSUPPORTED;
+    if ((double)g->nx * g->ny * g->nz >= 2147483647.0) return CTR_ERR_INVALID;
+    if ((double)g->n_views * g->nv * g->nu >= 2147483647.0) return CTR_ERR_INVALID;
+    if (g->nx > (1 << 24) || g->ny > (1 << 24) || g->nz > (1 << 24)) return CTR_ERR_INVALID;
+    const double tiles = (double)((g->nx + kPxTile - 1) / kPxTile) * ((g->ny + kPxTile - 1) / kPxTile);
+    if (tiles * ((g->nz + kPxTz - 1) / kPxTz) >= 2147483647.0) return CTR_ERR_INVALID;
+    if (tiles * g->n_views >= 2147483647.0) return CTR_ERR_INVALID;
+    // farthest voxel column from the rotation axis: R - s stays >= R/1000 for every view
+    const double xr = fabs(0.5 * (g->nx - 1) * g->dx) + fabs(g->recon_center[0]);
+    const double yr = fabs(0.5 * (g->ny - 1) * g->dy) + fabs(g->recon_center[1]);
+    if (!(g->sad - sqrt(xr * xr + yr * yr) >= 1e-3 * g->sad)) return CTR_ERR_UNSUPPORTED;
+    return CTR_OK;
+}
+
+}  // namespace
+
+size_t ctr_pixel_backproject_workspace_bytes(const ctr_geometry* g) {
+    if (check_pixel_geometry(g)) return 0;
+    return px_layout(g).total;
+}
+
+int ctr_pixel_backproject(const ctr_geometry* g, const double* angles, const float* q,
+                          float* vol, float scale, int accumulate, void* workspace,
+                          size_t workspace_bytes, void* stream) {
+    int st = check_pixel_geometry(g);
+    if (st) return st;
+    if (!angles || !q || !vol || !workspace) return CTR_ERR_INVALID;
+    if (!std::isfinite(scale)) return CTR_ERR_INVALID;
+    const PxLayout l = px_layout(g);
+    if (workspace_bytes < l.total) return CTR_ERR_WORKSPACE;
+    hipStream_t s = (hipStream_t)stream;
+    char* ws = (char*)workspace;
+    float2* cs = (float2*)(ws + l.cs_off);
+    float4* tb = (float4*)(ws + l.tb_off);
+    const double mag = g->sad / g->sdd;
+    const double x_first = -0.5 * (g->nx - 1) * g->dx + g->recon_center[0];
+    const double y_first = -0.5 * (g->ny - 1) * g->dy + g->recon_center[1];
+    const double z_first = -0.5 * (g->nz - 1) * g->dz + g->recon_center[2];
+    const double inv_dk = 1.0 / (g->du * mag), inv_dp = 1.0 / (g->dv * mag);
+    const int n_tiles_xy = l.tiles_x * l.tiles_y;
+
+    hipLaunchKernelGGL(px_view_kernel, dim3((g->n_views + 255) / 256), dim3(256), 0, s, angles,
+                       cs, g->n_views);
+    if ((st = status_from_hip(hipGetLastError()))) return st;
+    PxTileArgs ta;
+    ta.n_views = g->n_views;
+    ta.tiles_x = l.tiles_x;
+    ta.n_tiles_xy = n_tiles_xy;
+    ta.nx = g->nx;
+    ta.ny = g->ny;
+    ta.pad = 0;
+    ta.x_first = x_first;
+    ta.y_first = y_first;
+    ta.dx = g->dx;
+    ta.dy = g->dy;
+    ta.R = g->sad;
+    ta.inv_dk = inv_dk;
+    ta.u_mid = 0.5 * (g->nu - 1);
+    const int n_tb = g->n_views * n_tiles_xy;
+    hipLaunchKernelGGL(px_tile_kernel, dim3((n_tb + 255) / 256), dim3(256), 0, s, angles, tb, ta);
+    if ((st = status_from_hip(hipGetLastError()))) return st;
+
+    PxArgs a;
+    a.nx = g->nx;
+    a.ny = g->ny;
+    a.nz = g->nz;
+    a.nu = g->nu;
+    a.nv = g->nv;
+    a.n_views = g->n_views;
+    a.tiles_x = l.tiles_x;
+    a.tiles_y = l.tiles_y;
+    a.n_tiles_xy = n_tiles_xy;
+    a.accumulate = accumulate ? 1 : 0;
+    a.x_first = (float)x_first;
+    a.y_first = (float)y_first;
+    a.z_first = (float)z_first;
+    a.dx = (float)g->dx;
+    a.dy = (float)g->dy;
+    a.dz = (float)g->dz;
+    a.R = (float)g->sad;
+    a.inv_dk = (float)inv_dk;
+    a.u_mid = (float)(0.5 * (g->nu - 1));
+    a.u_last = (float)(g->nu - 1);
+    a.inv_dp = (float)inv_dp;
+    a.v_mid = (float)(0.5 * (g->nv - 1));
+    a.v_last = (float)(g->nv - 1);
+    a.scale = scale;
+    const dim3 grid((unsigned)(n_tiles_xy * l.slabs));
+    hipLaunchKernelGGL(pixel_backproject_kernel, grid, dim3(kPxThreads), 0, s, q, vol, cs, tb, a);
+    return status_from_hip(hipGetLastError());
+}

@@ -39,12 +39,14 @@ class ParamsFileError(ValueError):
     pass
 
 
-def read_params_file(filename: str, fan: bool = False) -> dict:
+def read_params_file(filename: str, fan: bool = False, keys: tuple | None = None) -> dict:
     """Parse a geometry text file into the ``ConeBeam`` (or ``FanBeam``) params dict.
 
     Every key starts at the reference's default (0 or ''); ``AngleCoverage`` is returned in
-    radians, as the reference stores it."""
-    keys = FAN_KEYS if fan else CONE_KEYS
+    radians, as the reference stores it.  ``keys`` replaces the accepted key set (the GPU
+    ``ConeBeam`` of python/ConeBeamGPU.py:164-168 also accepts ``Mode``)."""
+    if keys is None:
+        keys = FAN_KEYS if fan else CONE_KEYS
     params = {k: ("" if k in ("DataPath", "precision") else 0) for k in keys}
     with open(filename) as f:
         for lineno, line in enumerate(f, 1):

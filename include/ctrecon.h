@@ -132,6 +132,41 @@ int ctr_dd_backproject(const ctr_geometry* g, const double* angles, const int32_
                        const float* proj, float* vol, int mode, float scale, float zero_tol,
                        int accumulate, void* workspace, size_t workspace_bytes, void* stream);
 
+/* ---- Pixel-driven cone-beam FDK ----------------------------------------------------------
+ * Replaces ConeBeam.Reconstruction of python/ConeBeamGPU.py:203-321, the reference's
+ * voxel-driven FDK with bilinear detector interpolation.  Geometry fields used: nx, ny, nz,
+ * nu, nv, n_views, dx, dy, dz, du, dv, sad (R), sdd, recon_center; detector_shape must be
+ * flat and det_offset, helical zero (CTR_ERR_UNSUPPORTED otherwise); nu, nv >= 2.  Voxel
+ * (ix, iy, iz) sits at ((ix - (nx-1)/2) dx, (iy - (ny-1)/2) dy, (iz - (nz-1)/2) dz) +
+ * recon_center, every axis ascending: the reference's linspace(-fov/2, fov/2, n) grid when
+ * dx = fov/(n-1).  Detector cell (k, j) of a view sits at ((j - (nu-1)/2) du, (k - (nv-1)/2) dv)
+ * scaled by R/SDD to the rotation axis.
+ *
+ * ctr_pixel_filter_spectrum replaces ConeBeam.Filter (:355-384): 2 |DFT| of the Ram-Lak taps
+ * for pixel size tau, windowed, zero where |w| > pi*cutoff (w in rad/sample), in FFT order,
+ * float64[n_fft]; n_fft a power of two <= 8192.
+ *
+ * ctr_pixel_filter replaces the per-view filtering (:243, :296-302): weight
+ * R/sqrt(R^2 + kk^2 + pp^2) at cell centres, fliplr, zero-padded FFT convolution with
+ * `spectrum` (length ctr_zero_padded_length(nu)), crop to nu, no tau factor.  The output Q is
+ * the reference's: its columns are mirrored (Q[v][k][j] comes from input column nu-1-j).
+ * Workspace: ctr_fdk_filter_workspace_bytes(g).  proj_in must differ from q_out.
+ *
+ * ctr_pixel_backproject replaces CalculateInterpolGrid + Interpol2dLineargpu and the view loop
+ * (:65-129, :303-315): vol = (accumulate ? vol : 0) + sum_views scale * R^2/(R-s)^2 *
+ * bilinear(Q_view; R t/(R-s), R z/(R-s)) with t = x cos + y sin, s = -x sin + y cos and the
+ * sample 0 outside the detector grid (the reference's fill_value); the reference's scale is
+ * dtheta.  The source must stay outside the voxel grid (R - s > 0 everywhere). */
+int ctr_pixel_filter_spectrum(int n_fft, double tau, int filter_type, double cutoff,
+                              double* spectrum, void* stream);
+int ctr_pixel_filter(const ctr_geometry* g, const double* spectrum, const float* proj_in,
+                     float* q_out, int n_views, void* workspace, size_t workspace_bytes,
+                     void* stream);
+size_t ctr_pixel_backproject_workspace_bytes(const ctr_geometry* g);
+int ctr_pixel_backproject(const ctr_geometry* g, const double* angles, const float* q,
+                          float* vol, float scale, int accumulate, void* workspace,
+                          size_t workspace_bytes, void* stream);
+
 /* ---- Learned prior: 3x3 convolutions, bfloat16 on MFMA, FP32 accumulation ---------------
  * No counterpart in the reference (the learned residual prior is new in this build).
  * Activations are NHWC bfloat16 [n][h][w][c]; weights bfloat16 [cout][3][3][cin]; bias float32.

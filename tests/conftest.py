@@ -28,6 +28,18 @@ def make_params(**kw):
     return p
 
 
+def make_cone_params(**kw):
+    """``ConeBeam`` params dict (python/ConeBeamGPU.py:164-197; AngleCoverage in radians)."""
+    p = {
+        "SourceToAxis": 100.0, "SourceToDetector": 150.0, "DetectorPixelWidth": 0.8,
+        "DetectorPixelHeight": 0.8, "DetectorWidth": 48, "DetectorHeight": 40,
+        "NumberOfViews": 48, "AngleCoverage": 2 * np.pi, "ReconX": 24, "ReconY": 24,
+        "ReconZ": 24, "DataPath": "", "precision": "float32",
+    }
+    p.update(kw)
+    return p
+
+
 @pytest.fixture
 def params_factory():
     return make_params

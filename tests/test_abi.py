@@ -139,3 +139,61 @@ def test_product_path_fails_loudly_without_gpu_tensors():
     geom = ScanGeometry(make_params())
     with pytest.raises(ValueError, match="GPU"):
         P.dd_forward(torch.zeros(geom.vol_shape), geom)
+
+
+def _pixel_struct(**kw):
+    from ctreconstruction_amd.pixel_fdk import PixelScan
+    from tests.conftest import make_cone_params
+    return PixelScan(make_cone_params(**kw)).ctypes_struct()
+
+
+def test_pixel_backproject_rejects_unsupported_geometry_before_launch(lib):
+    dummy = ctypes.c_void_p(16)  # never dereferenced: validation fails first
+
+    def bp(g, ws=1 << 30):
+        return lib.ctr_pixel_backproject(ctypes.byref(g), dummy, dummy, dummy, 1.0, 0, dummy, ws, None)
+
+    g = _pixel_struct()
+    assert lib.ctr_pixel_backproject_workspace_bytes(ctypes.byref(g)) > 0
+    g.detector_shape = N.CTR_DETECTOR_CURVED
+    assert bp(g) == N.CTR_ERR_UNSUPPORTED
+    g = _pixel_struct()
+    g.det_offset[0] = 0.5
+    assert bp(g) == N.CTR_ERR_UNSUPPORTED
+    g = _pixel_struct()
+    g.nv = 1  # the reference indexes p[1]
+    assert bp(g) == N.CTR_ERR_INVALID
+    assert lib.ctr_pixel_backproject_workspace_bytes(ctypes.byref(g)) == 0
+    g = _pixel_struct()
+    g.sad = 10.0  # source inside the voxel grid
+    assert bp(g) == N.CTR_ERR_UNSUPPORTED
+    g = _pixel_struct()
+    need = lib.ctr_pixel_backproject_workspace_bytes(ctypes.byref(g))
+    assert bp(g, need - 1) == N.CTR_ERR_WORKSPACE
+    assert lib.ctr_pixel_backproject(ctypes.byref(g), dummy, dummy, dummy, float("nan"), 0, dummy,
+                                     need, None) == N.CTR_ERR_INVALID
+
+
+def test_pixel_filter_arguments_are_rejected_before_launch(lib):
+    dummy = ctypes.c_void_p(16)
+    assert lib.ctr_pixel_filter_spectrum(48, 0.5, 0, 0.3, dummy, None) == N.CTR_ERR_INVALID  # not 2^k
+    assert lib.ctr_pixel_filter_spectrum(16384, 0.5, 0, 0.3, dummy, None) == N.CTR_ERR_UNSUPPORTED
+    assert lib.ctr_pixel_filter_spectrum(64, 0.0, 0, 0.3, dummy, None) == N.CTR_ERR_INVALID
+    assert lib.ctr_pixel_filter_spectrum(64, 0.5, 4, 0.0, dummy, None) == N.CTR_ERR_INVALID
+    g = _pixel_struct()
+    # in place is refused: rows are written mirrored
+    assert lib.ctr_pixel_filter(ctypes.byref(g), dummy, dummy, dummy, 1, dummy, 1 << 30, None) == \
+        N.CTR_ERR_INVALID
+
+
+def test_pixel_scan_places_voxels_on_the_reference_linspace():
+    from ctreconstruction_amd.pixel_fdk import PixelScan
+    from oracle import pixel_oracle as PO
+    from tests.conftest import make_cone_params
+    for kw in ({}, {"ReconX": 7, "ReconY": 1, "ReconZ": 5}):
+        prm = make_cone_params(**kw)
+        g = PixelScan(prm).ctypes_struct()
+        ref = PO.PixelGeometry(prm)
+        for n, d, c, axis in ((g.nx, g.dx, g.recon_center[0], ref.x), (g.ny, g.dy, g.recon_center[1], ref.y),
+                              (g.nz, g.dz, g.recon_center[2], ref.z)):
+            np.testing.assert_allclose((np.arange(n) - (n - 1) / 2) * d + c, axis, rtol=0, atol=1e-12)
