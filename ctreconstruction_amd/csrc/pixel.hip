@@ -11,11 +11,12 @@
 //   px_view_kernel               cos/sin per view, FP64 -> FP32
 //   px_tile_kernel               per (view, 16x16 column tile): range of the detector column
 //                                coordinate and of the magnification R/(R-s) over the tile corners
-//   pixel_backproject_kernel     one workgroup per 16 x 16 x kPxTz voxel tile, one work-item per
-//                                (x, y) column holding kPxTz accumulators; per view the tile's
-//                                detector window is staged in LDS and every voxel reads its four
-//                                samples there (two ds_read2); windows wider than the LDS tile fall
-//                                back to direct reads from global memory for that view.
+//   pixel_backproject_kernel     one workgroup per 16 x 16 x TZ voxel tile, one work-item per
+//                                (x, y) column holding TZ accumulators; per view the tile's
+//                                detector window is staged in LDS (double-buffered, fetched a view
+//                                ahead) and every voxel reads its four samples there (two
+//                                ds_read2); windows larger than the LDS tile fall back to direct
+//                                reads from global memory for that view.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -26,9 +27,10 @@
 namespace {
 
 constexpr int kPxTile = 16;     // in-plane tile edge (voxels)
-constexpr int kPxTz = 32;       // z voxels per work-item
+constexpr int kPxTz = 32;       // z voxels per work-item (kPxTzShort where windows are taller)
+constexpr int kPxTzShort = 16;
 constexpr int kPxPitch = 65;    // LDS window row pitch (floats); a window has <= 64 columns
-constexpr int kPxRows = 96;     // LDS window rows
+constexpr int kPxRows = 72;     // LDS window rows (two windows per group: 37 KB, 4 groups per CU)
 constexpr int kPxThreads = kPxTile * kPxTile;
 
 size_t align_up(size_t n, size_t a) { return (n + a - 1) / a * a; }
@@ -87,12 +89,45 @@ __global__ void px_tile_kernel(const double* __restrict__ angles, float4* __rest
     tb[idx] = make_float4((float)fu_lo, (float)fu_hi, (float)m_lo, (float)m_hi);
 }
 
+// Window of a (tile, view) pair: detector columns floor(fu_lo)-1 .. floor(fu_hi)+2 and rows
+// likewise from the extremes of V over the tile's z range (all values uniform in the group).
+struct PxWindow {
+    int c0, cols, r0, rows;
+};
+
+template <int ROWS>
+__device__ inline PxWindow px_window(const float4 bnd, float zlo, float zhi, const PxArgs& a) {
+    PxWindow w;
+    w.c0 = (int)floorf(bnd.x) - 1;
+    w.cols = (int)floorf(bnd.y) + 3 - w.c0;
+    const float va = bnd.z * zlo, vb = bnd.w * zlo, vc = bnd.z * zhi, vd = bnd.w * zhi;
+    const float fv_lo = fmaf(fminf(fminf(va, vb), fminf(vc, vd)), a.inv_dp, a.v_mid);
+    const float fv_hi = fmaf(fmaxf(fmaxf(va, vb), fmaxf(vc, vd)), a.inv_dp, a.v_mid);
+    w.r0 = (int)floorf(fv_lo) - 1;
+    w.rows = (int)floorf(fv_hi) + 3 - w.r0;
+    return w;
+}
+
+template <int ROWS>
+__device__ inline bool px_fits(const PxWindow& w) {
+    return w.cols <= kPxPitch - 1 && w.rows <= ROWS;
+}
+
+// One group per 16 x 16 x TZ voxel tile.  Per view, the window rows are fetched with one buffer
+// load per (row, column) into registers a view ahead, written to one of two LDS buffers and
+// read back by the voxels; a single barrier per view separates the write of a buffer from its
+// reads two views earlier.  Cells outside the detector get a buffer offset past the view and
+// the range check returns 0 for them.
+template <int TZ, int ROWS>
 __global__ void __launch_bounds__(kPxThreads) pixel_backproject_kernel(
     const float* __restrict__ q, float* __restrict__ vol, const float2* __restrict__ cs,
     const float4* __restrict__ tb, PxArgs a) {
-    __shared__ float win[kPxRows * kPxPitch];
+    constexpr int kBuf = ROWS * kPxPitch;
+    constexpr int kStage = ROWS / 4;
+    __shared__ float win[2 * kBuf];
     const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int tx = tid & (kPxTile - 1), ty = tid / kPxTile;
     int b = blockIdx.x;
     const int bx = b % a.tiles_x;
@@ -100,116 +135,125 @@ __global__ void __launch_bounds__(kPxThreads) pixel_backproject_kernel(
     const int by = b % a.tiles_y;
     const int bz = b / a.tiles_y;
     const int tile = by * a.tiles_x + bx;
-    const int ix = bx * kPxTile + tx, iy = by * kPxTile + ty, iz0 = bz * kPxTz;
+    const int ix = bx * kPxTile + tx, iy = by * kPxTile + ty, iz0 = bz * TZ;
     // past the grid's x/y end a work-item samples at the last voxel (its sums are not stored);
     // past its z end the nominal positions are kept, which only lengthens the window in v
     const float x = fmaf((float)min(ix, a.nx - 1), a.dx, a.x_first);
     const float y = fmaf((float)min(iy, a.ny - 1), a.dy, a.y_first);
     const float zlo = fmaf((float)iz0, a.dz, a.z_first);
-    const float zhi = fmaf((float)(iz0 + kPxTz - 1), a.dz, a.z_first);
+    const float zhi = fmaf((float)(iz0 + TZ - 1), a.dz, a.z_first);
     const size_t view_cells = (size_t)a.nu * a.nv;
+    const int view_bytes = (int)(view_cells * 4);
 
-    float acc[kPxTz];
+    float acc[TZ];
 #pragma unroll
-    for (int k = 0; k < kPxTz; ++k) acc[k] = 0.0f;
+    for (int k = 0; k < TZ; ++k) acc[k] = 0.0f;
+
+    float stage[kStage];
+    // issue the window loads of view v into `stage` (rows wave, wave + 4, ...)
+    auto fetch = [&](int v, const PxWindow& w) {
+        const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)(q + (size_t)v * view_cells), 0, view_bytes, 0x00020000);
+        const int gc = w.c0 + lane;
+        const unsigned lane_off =
+            (lane < w.cols && (unsigned)gc < (unsigned)a.nu) ? 4u * (unsigned)gc : 0x7ffffff0u;
+#pragma unroll
+        for (int i = 0; i < kStage; ++i) {
+            const int r = i * 4 + wave;
+            if (r < w.rows) {
+                const int gr = w.r0 + r;
+                const unsigned row_off =
+                    (unsigned)gr < (unsigned)a.nv ? 4u * (unsigned)(gr * a.nu) : 0x7ffffff0u;
+                stage[i] = __uint_as_float(
+                    __builtin_amdgcn_raw_buffer_load_b32(rsrc, (int)(lane_off + row_off), 0, 0));
+            }
+        }
+    };
+    auto put = [&](float* buf, const PxWindow& w) {
+#pragma unroll
+        for (int i = 0; i < kStage; ++i) {
+            const int r = i * 4 + wave;
+            if (r < w.rows) buf[r * kPxPitch + lane] = stage[i];
+        }
+    };
+
+    PxWindow wn = px_window<ROWS>(tb[tile], zlo, zhi, a);
+    bool fits_n = px_fits<ROWS>(wn);
+    if (fits_n) fetch(0, wn);
 
     for (int v = 0; v < a.n_views; ++v) {
-        const float2 sc = cs[v];
-        const float4 bnd = tb[(size_t)v * a.n_tiles_xy + tile];
-        // window: columns floor(fu_lo)-1 .. floor(fu_hi)+2, rows likewise from the V extremes
-        const int c0 = (int)floorf(bnd.x) - 1;
-        const int cols = (int)floorf(bnd.y) + 3 - c0;
-        const float va = bnd.z * zlo, vb = bnd.w * zlo, vc = bnd.z * zhi, vd = bnd.w * zhi;
-        const float fv_lo = fmaf(fminf(fminf(va, vb), fminf(vc, vd)), a.inv_dp, a.v_mid);
-        const float fv_hi = fmaf(fmaxf(fmaxf(va, vb), fmaxf(vc, vd)), a.inv_dp, a.v_mid);
-        const int r0 = (int)floorf(fv_lo) - 1;
-        const int rows = (int)floorf(fv_hi) + 3 - r0;
-        const bool fits = cols <= kPxPitch - 1 && rows <= kPxRows;
+        const PxWindow w = wn;
+        const bool fits = fits_n;
+        float* buf = win + (v & 1) * kBuf;
+        if (fits) put(buf, w);
+        if (v + 1 < a.n_views) {
+            wn = px_window<ROWS>(tb[(size_t)(v + 1) * a.n_tiles_xy + tile], zlo, zhi, a);
+            fits_n = px_fits<ROWS>(wn);
+            if (fits_n) fetch(v + 1, wn);
+        }
+        __syncthreads();
 
         // this column's sample geometry (python/ConeBeamGPU.py:125-127)
+        const float2 sc = cs[v];
         const float t = fmaf(x, sc.x, y * sc.y);
         const float s = fmaf(y, sc.x, -x * sc.y);
-        const float m = a.R / (a.R - s);
+        const float m = a.R * __builtin_amdgcn_rcpf(a.R - s);  // R/(R-s), 1 ulp
         const float fu = fmaf(t * m, a.inv_dk, a.u_mid);
         const bool uok = fu >= 0.0f && fu <= a.u_last;
-        const float w = m * m * a.scale;
-        float kz = m * a.dz * a.inv_dp;
-        float fvb = fmaf(m * zlo, a.inv_dp, a.v_mid);
+        const float kz = m * a.dz * a.inv_dp;
+        const float fvb = fmaf(m * zlo, a.inv_dp, a.v_mid);
+        // iu = floor(fu) except on the last grid column (fu = nu-1: iu = nu-2, fx = 1)
+        const int iu = min((int)floorf(fu), a.nu - 2);
+        const float fx = fu - (float)iu;
 
         if (fits) {
-            const float* qv = q + (size_t)v * view_cells;
-            __syncthreads();  // the previous view's window is no longer read
-            float stage[kPxRows / 4];
+            // every row floor(fv(k)) must lie in [r0, r0 + rows - 2]; fv is monotone in k
+            const float f_last = fmaf((float)(TZ - 1), kz, fvb);
+            const bool in_win = floorf(fminf(fvb, f_last)) >= (float)w.r0 &&
+                                floorf(fmaxf(fvb, f_last)) <= (float)(w.r0 + w.rows - 2);
+            const float wc = (uok && in_win) ? m * m * a.scale : 0.0f;
+            const float kz_c = in_win ? kz : 0.0f;
+            const float fvb_c = in_win ? fvb : (float)w.r0;
+            const int cu = min(max(iu - w.c0, 0), w.cols - 2);
+            const float* base = buf + cu - w.r0 * kPxPitch;
 #pragma unroll
-            for (int i = 0; i < kPxRows / 4; ++i) {
-                const int r = i * 4 + wave;
-                const int gr = r0 + r, gc = c0 + lane;
-                float val = 0.0f;
-                if (r < rows && lane < cols && (unsigned)gr < (unsigned)a.nv &&
-                    (unsigned)gc < (unsigned)a.nu)
-                    val = qv[(size_t)gr * a.nu + gc];
-                stage[i] = val;
-            }
-#pragma unroll
-            for (int i = 0; i < kPxRows / 4; ++i) {
-                const int r = i * 4 + wave;
-                if (r < rows && lane < cols) win[r * kPxPitch + lane] = stage[i];
-            }
-            __syncthreads();
-
-            // iu = floor(fu) except on the last grid column (fu = nu-1: iu = nu-2, fx = 1)
-            const int iu = min((int)floorf(fu), a.nu - 2);
-            const float fx = fu - (float)iu;
-            const int cu = min(max(iu - c0, 0), cols - 2);
-            // every row index floor(fv(k)) must lie in [r0, r0 + rows - 2]; fv is monotone in k
-            const float f_first = fvb;
-            const float f_last = fmaf((float)(kPxTz - 1), kz, fvb);
-            const bool in_win = floorf(fminf(f_first, f_last)) >= (float)r0 &&
-                                floorf(fmaxf(f_first, f_last)) <= (float)(r0 + rows - 2);
-            const bool ok_col = uok && in_win;
-            if (!in_win) {
-                fvb = (float)r0;
-                kz = 0.0f;
-            }
-            const int colterm = cu - r0 * kPxPitch;
-#pragma unroll
-            for (int k = 0; k < kPxTz; ++k) {
-                const float fv = fmaf((float)k, kz, fvb);
-                const float fl = floorf(fv);
-                const float fy = fv - fl;
-                const int addr = (int)fl * kPxPitch + colterm;
-                const float q00 = win[addr], q01 = win[addr + 1];
-                const float q10 = win[addr + kPxPitch], q11 = win[addr + kPxPitch + 1];
+            for (int k = 0; k < TZ; ++k) {
+                const float fv = fmaf((float)k, kz_c, fvb_c);
+                const int iv = (int)floorf(fv);
+                const float fy = __builtin_amdgcn_fractf(fv);
+                const float* p = base + __mul24(iv, kPxPitch);
+                const float q00 = p[0], q01 = p[1], q10 = p[kPxPitch], q11 = p[kPxPitch + 1];
                 const float m0 = fmaf(fx, q01 - q00, q00);
                 const float m1 = fmaf(fx, q11 - q10, q10);
                 const float val = fmaf(fy, m1 - m0, m0);
-                const bool ok = ok_col && fv >= 0.0f && fv <= a.v_last;
-                acc[k] = ok ? fmaf(val, w, acc[k]) : acc[k];
+                const float wk = (fv >= 0.0f && fv <= a.v_last) ? wc : 0.0f;
+                acc[k] = fmaf(val, wk, acc[k]);
             }
         } else {
-            // window wider than the LDS tile: the four samples come from global memory
+            // window larger than the LDS tile: the four samples come from global memory
             const float* qv = q + (size_t)v * view_cells;
-            const int iu = min(max((int)floorf(fu), 0), a.nu - 2);
-            const float fx = fu - (float)iu;
-#pragma unroll 4
-            for (int k = 0; k < kPxTz; ++k) {
+            const int iuc = max(iu, 0);
+            const float fxc = __saturatef(fu - (float)iuc);
+            const float wc = uok ? m * m * a.scale : 0.0f;
+#pragma unroll
+            for (int k = 0; k < TZ; ++k) {
                 const float fv = fmaf((float)k, kz, fvb);
-                const bool ok = uok && fv >= 0.0f && fv <= a.v_last;
-                const int iv = ok ? min((int)floorf(fv), a.nv - 2) : 0;
-                const float fy = fv - (float)iv;
-                const float* r0p = qv + (size_t)iv * a.nu + iu;
+                const bool ok = fv >= 0.0f && fv <= a.v_last;
+                const int iv = min(max((int)floorf(fv), 0), a.nv - 2);
+                const float fy = __saturatef(fv - (float)iv);
+                const float* r0p = qv + (size_t)iv * a.nu + iuc;
                 const float q00 = r0p[0], q01 = r0p[1], q10 = r0p[a.nu], q11 = r0p[a.nu + 1];
-                const float m0 = fmaf(fx, q01 - q00, q00);
-                const float m1 = fmaf(fx, q11 - q10, q10);
+                const float m0 = fmaf(fxc, q01 - q00, q00);
+                const float m1 = fmaf(fxc, q11 - q10, q10);
                 const float val = fmaf(fy, m1 - m0, m0);
-                acc[k] = ok ? fmaf(val, w, acc[k]) : acc[k];
+                acc[k] = fmaf(val, ok ? wc : 0.0f, acc[k]);
             }
         }
     }
 
     if (ix < a.nx && iy < a.ny) {
 #pragma unroll
-        for (int k = 0; k < kPxTz; ++k) {
+        for (int k = 0; k < TZ; ++k) {
             const int iz = iz0 + k;
             if (iz < a.nz) {
                 const size_t o = ((size_t)iz * a.ny + iy) * a.nx + ix;
@@ -221,14 +265,31 @@ __global__ void __launch_bounds__(kPxThreads) pixel_backproject_kernel(
 
 struct PxLayout {
     size_t cs_off, tb_off, total;
-    int tiles_x, tiles_y, slabs;
+    int tiles_x, tiles_y, slabs, tz;
 };
+
+// z voxels per work-item: kPxTz when the window of the outermost tile (largest magnification
+// m_max = R/(R - r_max), its change across a tile, the largest |z|) stays within kPxRows rows,
+// else kPxTzShort.  (Either way a (tile, view) window that does not fit is read from global
+// memory.)
+int px_choose_tz(const ctr_geometry* g) {
+    const double xr = fabs(0.5 * (g->nx - 1) * g->dx) + fabs(g->recon_center[0]);
+    const double yr = fabs(0.5 * (g->ny - 1) * g->dy) + fabs(g->recon_center[1]);
+    const double m_max = g->sad / (g->sad - sqrt(xr * xr + yr * yr));
+    const double dp = g->dv * g->sad / g->sdd;
+    const double z_max = fabs(0.5 * (g->nz - 1) * g->dz) + fabs(g->recon_center[2]);
+    const double tile_span = kPxTile * sqrt(g->dx * g->dx + g->dy * g->dy);
+    const double rows = (kPxTz - 1) * g->dz * m_max / dp +
+                        m_max * m_max * tile_span / g->sad * z_max / dp + 5.0;
+    return rows <= kPxRows ? kPxTz : kPxTzShort;
+}
 
 PxLayout px_layout(const ctr_geometry* g) {
     PxLayout l;
     l.tiles_x = (g->nx + kPxTile - 1) / kPxTile;
     l.tiles_y = (g->ny + kPxTile - 1) / kPxTile;
-    l.slabs = (g->nz + kPxTz - 1) / kPxTz;
+    l.tz = px_choose_tz(g);
+    l.slabs = (g->nz + l.tz - 1) / l.tz;
     l.cs_off = 0;
     l.tb_off = align_up((size_t)g->n_views * sizeof(float2), 256);
     l.total = l.tb_off + align_up((size_t)g->n_views * l.tiles_x * l.tiles_y * sizeof(float4), 256);
@@ -252,7 +313,8 @@ int check_pixel_geometry(const ctr_geometry* g) {
     if ((double)g->n_views * g->nv * g->nu >= 2147483647.0) return CTR_ERR_INVALID;
     if (g->nx > (1 << 24) || g->ny > (1 << 24) || g->nz > (1 << 24)) return CTR_ERR_INVALID;
     const double tiles = (double)((g->nx + kPxTile - 1) / kPxTile) * ((g->ny + kPxTile - 1) / kPxTile);
-    if (tiles * ((g->nz + kPxTz - 1) / kPxTz) >= 2147483647.0) return CTR_ERR_INVALID;
+    if (tiles * ((g->nz + kPxTzShort - 1) / kPxTzShort) >= 2147483647.0) return CTR_ERR_INVALID;
+    if ((double)g->nu * g->nv * 4.0 >= 2147483632.0) return CTR_ERR_INVALID;  // buffer offsets
     if (tiles * g->n_views >= 2147483647.0) return CTR_ERR_INVALID;
     // farthest voxel column from the rotation axis: R - s stays >= R/1000 for every view
     const double xr = fabs(0.5 * (g->nx - 1) * g->dx) + fabs(g->recon_center[0]);
@@ -335,6 +397,11 @@ int ctr_pixel_backproject(const ctr_geometry* g, const double* angles, const flo
     a.v_last = (float)(g->nv - 1);
     a.scale = scale;
     const dim3 grid((unsigned)(n_tiles_xy * l.slabs));
-    hipLaunchKernelGGL(pixel_backproject_kernel, grid, dim3(kPxThreads), 0, s, q, vol, cs, tb, a);
+    if (l.tz == kPxTz)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(pixel_backproject_kernel<kPxTz, kPxRows>), grid,
+                           dim3(kPxThreads), 0, s, q, vol, cs, tb, a);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(pixel_backproject_kernel<kPxTzShort, kPxRows>), grid,
+                           dim3(kPxThreads), 0, s, q, vol, cs, tb, a);
     return status_from_hip(hipGetLastError());
 }
