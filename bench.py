@@ -14,6 +14,11 @@ with algorithmic bytes 4*(views*nv*nu + nx*ny*nz) per launch; ``cpu_baseline`` t
 NumPy oracle on a bounded sample spread over one process per host core (two views per process,
 central 16-slice slab) and scales the wall time to a full reconstruction; a single-thread run
 (2 views, central 8-slice slab) is reported beside it.
+
+``pixel_fdk`` times the reference's other FDK, the pixel-driven one (voxel-driven bilinear
+backprojection, python/ConeBeamGPU.py:203-321), on the reference's geometry file
+(python/ReconstructionParams.txt: 512^3, 680 views, 1024 x 768 detector at 0.390625 mm), with
+its own single-thread FP64 oracle sample beside it.
 """
 from __future__ import annotations
 
@@ -33,6 +38,14 @@ HBM_PEAK_GBS = 8000.0
 # launches a flag fill and the remainder pass, both under 1% of it for this workload; the HIP
 # events bracket all three, so the achieved rate is a lower bound for the pass itself)
 BP_KERNEL = "dd_backproject_tile_kernel<16, 0>"
+
+# The reference's own geometry file for its pixel-driven FDK (python/ReconstructionParams.txt:1-11,
+# read by python/ConeBeamGPU.py:164-197); AngleCoverage in radians as the reader stores it.
+PIXEL_GEOMETRY = dict(SourceToAxis=1000.0, SourceToDetector=1500.0, DetectorPixelWidth=0.390625,
+                      DetectorPixelHeight=0.390625, DetectorWidth=1024, DetectorHeight=768,
+                      NumberOfViews=680, AngleCoverage=2 * np.pi, ReconX=512, ReconY=512, ReconZ=512,
+                      DataPath="", precision="float32")
+PIXEL_KERNEL = "pixel_backproject_kernel<32, 72>"
 
 CONFIGS = {
     "c3": dict(n=512, voxel=0.25, views=720, nu=1024, nv=768, pitch=0.4, ellipsoids=40, seed=3,
@@ -166,6 +179,81 @@ def cpu_baseline(params, q_gpu, geom, dev, pool=None, sample_views=(0, 1), slab=
 WEIGHTS = os.path.join(ROOT, "ctreconstruction_amd", "weights", "prior_sparse60.safetensors")
 
 
+def pixel_fdk_leg(dev, stream, steps, warmup, cpu=True, seed=3):
+    """Pixel-driven FDK (``ConeBeam.Reconstruction``, python/ConeBeamGPU.py:203-321) on the
+    reference's geometry file: weight + Ram-Lak (cutoff 0.3) filter of every row, then the
+    voxel-driven bilinear backprojection of 680 views into 512^3, event-timed on the stream.
+    Projections: the seeded random-ellipsoid phantom, DD-forward-projected on the same geometry.
+    ``cpu``: the FP64 oracle on one thread, one view filtered and two views backprojected into
+    the central 8 slices, scaled to the full reconstruction."""
+    import torch
+
+    import ctreconstruction_amd as ctr
+    from ctreconstruction_amd import paramsfile, phantom
+    from ctreconstruction_amd import pixel_fdk as PF
+
+    scan = PF.PixelScan(PIXEL_GEOMETRY)
+    geom = ctr.ScanGeometry(paramsfile.to_recon_params(PIXEL_GEOMETRY))
+    ell = phantom.random_ellipsoids(40, phantom.half_extent(geom), seed)
+    vol = phantom.ellipsoid_volume(geom, ell, device=dev) * phantom.MU_WATER
+    proj = ctr.dd_forward(vol, geom)
+    del vol
+    spec = PF.pixel_spectrum(scan.zero_padded_length, scan.tau, device=dev)
+    q = torch.empty_like(proj)
+    rec = torch.empty(scan.vol_shape, dtype=torch.float32, device=dev)
+    ang = scan.device_angles(dev)
+
+    def step(ev=None):
+        if ev is not None:
+            ev[0].record(stream)
+        PF.pixel_filter(proj, scan, spec, out=q)
+        if ev is not None:
+            ev[1].record(stream)
+        PF.pixel_backproject(q, scan, out=rec, angles=ang)
+        if ev is not None:
+            ev[2].record(stream)
+
+    for _ in range(warmup):
+        step()
+    evs = [tuple(torch.cuda.Event(enable_timing=True) for _ in range(3)) for _ in range(steps)]
+    for k in range(steps):
+        step(evs[k])
+    torch.cuda.synchronize()
+    f_s = float(np.mean([a.elapsed_time(b) for a, b, _ in evs])) / 1e3
+    b_s = float(np.mean([b.elapsed_time(c) for _, b, c in evs])) / 1e3
+    updates = float(scan.nx * scan.ny * scan.nz) * scan.n_views
+    prof = profile_record("pixel_backproject_traffic", PIXEL_KERNEL)
+    out = {"workload": "pixel-driven FDK 512^3, 680 views, 1024x768 detector (python/ReconstructionParams.txt)",
+           "s": f_s + b_s, "filter_s": f_s, "backprojection_s": b_s, "gups": updates / b_s / 1e9,
+           "kernel": PIXEL_KERNEL, "valu_busy": prof.get("valu_busy_frac"),
+           "hbm_traffic_bytes": prof.get("traffic_bytes")}
+    if cpu:
+        from oracle import pixel_oracle as PO
+        og = PO.PixelGeometry(PIXEL_GEOMETRY)
+        raw = proj[:2].cpu().numpy().astype(np.float64)
+        t0 = time.perf_counter()
+        qs = PO.filter_projections(og, raw[:1])
+        t_f = time.perf_counter() - t0
+        qs = PO.filter_projections(og, raw)
+        slab = 8
+        og.z = og.z[og.nz // 2 - slab // 2: og.nz // 2 + slab // 2]
+        og.nz = slab
+        t0 = time.perf_counter()
+        cpu_vol = PO.backproject(og, qs, views=(0, 1))
+        t_b = time.perf_counter() - t0
+        full = t_f * scan.n_views + t_b / (2 * slab) * scan.n_views * scan.nz
+        gpu_slab = PF.pixel_backproject(q[:2].contiguous(), PF.PixelScan({**PIXEL_GEOMETRY, "NumberOfViews": 2}),
+                                        scale=scan.dtheta, angles=ang[:2].contiguous())
+        # the two-view scan has its own angle grid; compare the slab sums with the same two angles
+        g_slab = gpu_slab[scan.nz // 2 - slab // 2: scan.nz // 2 + slab // 2].cpu().numpy()
+        out["cpu_baseline"] = {"value": full, "unit": "s", "cores": 1, "kind": "port",
+                               "sample": "FP64 oracle: filter of 1 view, backprojection of 2 views into the "
+                                         "central 8 slices, scaled to 680 views x 512 slices",
+                               "sample_max_abs_err_vs_gpu": float(np.abs(g_slab - cpu_vol).max()),
+                               "sample_peak": float(np.abs(cpu_vol).max())}
+    return out
+
+
 def learned_vs_sart(seed=10000):
     """Sparse-view benchmark (256^3, 60 views, 384^2, Poisson 1e5, held-out phantom): wall time
     and RMSE vs the phantom of plain 200-iteration SART and of the learned prior + SART pipeline
@@ -188,6 +276,7 @@ def main(argv=None):
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-prior time-to-RMSE leg")
+    ap.add_argument("--no-pixel", action="store_true", help="skip the pixel-driven FDK leg")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (rocprof counter calibration)")
     args = ap.parse_args(argv)
@@ -297,6 +386,8 @@ def main(argv=None):
                        "hbm_traffic_bytes": fp_prof.get("traffic_bytes"),
                        "texture_busy": {"ta": fp_prof.get("ta_busy_frac"), "td": fp_prof.get("td_busy_frac")}},
         "sart_vs_learned_time_to_rmse": None if args.no_learned else learned_vs_sart(),
+        "pixel_fdk": None if args.no_pixel else pixel_fdk_leg(dev, stream, args.steps, args.warmup,
+                                                             cpu=not args.no_cpu_baseline),
         "setup_s": t_setup,
     }
     if not args.no_cpu_baseline:

@@ -11,12 +11,14 @@
 //   px_view_kernel               cos/sin per view, FP64 -> FP32
 //   px_tile_kernel               per (view, 16x16 column tile): range of the detector column
 //                                coordinate and of the magnification R/(R-s) over the tile corners
+//   px_fill_kernel               zeroes the per-tile "has views for the rest pass" flags
 //   pixel_backproject_kernel     one workgroup per 16 x 16 x TZ voxel tile, one work-item per
 //                                (x, y) column holding TZ accumulators; per view the tile's
 //                                detector window is staged in LDS (double-buffered, fetched a view
 //                                ahead) and every voxel reads its four samples there (two
-//                                ds_read2); windows larger than the LDS tile fall back to direct
-//                                reads from global memory for that view.
+//                                ds_read2)
+//   pixel_backproject_rest_kernel   the (tile, view) pairs whose window is larger than the LDS
+//                                tile, read straight from global memory and added to the volume
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -29,7 +31,10 @@ namespace {
 constexpr int kPxTile = 16;     // in-plane tile edge (voxels)
 constexpr int kPxTz = 32;       // z voxels per work-item (kPxTzShort where windows are taller)
 constexpr int kPxTzShort = 16;
-constexpr int kPxPitch = 65;    // LDS window row pitch (floats); a window has <= 64 columns
+#ifndef CTR_PX_PITCH
+#define CTR_PX_PITCH 65
+#endif
+constexpr int kPxPitch = CTR_PX_PITCH;  // LDS window row pitch (floats); a window has <= 64 columns
 constexpr int kPxRows = 72;     // LDS window rows (two windows per group: 37 KB, 4 groups per CU)
 constexpr int kPxThreads = kPxTile * kPxTile;
 
@@ -45,8 +50,8 @@ struct PxArgs {
 };
 
 struct PxTileArgs {
-    int n_views, tiles_x, n_tiles_xy, nx, ny, pad;
-    double x_first, y_first, dx, dy, R, inv_dk, u_mid;
+    int n_views, tiles_x, n_tiles_xy, nx, ny, tz, slabs, rows_cap;
+    double x_first, y_first, z_first, dx, dy, dz, R, inv_dk, u_mid, inv_dp, v_mid;
 };
 
 __global__ void px_view_kernel(const double* __restrict__ angles, float2* __restrict__ cs,
@@ -61,8 +66,14 @@ __global__ void px_view_kernel(const double* __restrict__ angles, float2* __rest
 // U and R/(R-s) are monotone along every line of the (x, y) plane while R - s > 0, so their
 // extremes over a tile are at its corner voxel centres.  A tile that overhangs the grid is cut
 // at the last voxel: the work-items past it sample at that voxel's position.
+//
+// The same pass decides, in FP64 and once for both backprojection kernels, whether the
+// (tile, view) window fits the LDS tile for every z slab: the row span m z / dp over a slab
+// grows with |z|, so the first and the last slab bound it.  Two cells of margin in each
+// direction cover the float32 recomputation of the window in the main kernel.  A window that
+// does not fit is marked by a negative m_lo, and its tile by rest[tile] = 1.
 __global__ void px_tile_kernel(const double* __restrict__ angles, float4* __restrict__ tb,
-                               PxTileArgs a) {
+                               int* __restrict__ rest, PxTileArgs a) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= a.n_views * a.n_tiles_xy) return;
     const int v = idx / a.n_tiles_xy;
@@ -86,7 +97,23 @@ __global__ void px_tile_kernel(const double* __restrict__ angles, float4* __rest
             m_lo = fmin(m_lo, m);
             m_hi = fmax(m_hi, m);
         }
-    tb[idx] = make_float4((float)fu_lo, (float)fu_hi, (float)m_lo, (float)m_hi);
+    const double cols = floor(fu_hi) - floor(fu_lo) + 4.0;
+    double rows = 0.0;
+    for (int e = 0; e < 2; ++e) {
+        const double zlo = a.z_first + (double)((e ? a.slabs - 1 : 0) * a.tz) * a.dz;
+        const double zhi = zlo + (double)(a.tz - 1) * a.dz;
+        const double lo = (zlo < 0.0 ? m_hi : m_lo) * zlo * a.inv_dp + a.v_mid;
+        const double hi = (zhi >= 0.0 ? m_hi : m_lo) * zhi * a.inv_dp + a.v_mid;
+        rows = fmax(rows, floor(hi) - floor(lo) + 4.0);
+    }
+    const bool fits = cols <= (double)(kPxPitch - 1 - 2) && rows <= (double)(a.rows_cap - 2);
+    tb[idx] = make_float4((float)fu_lo, (float)fu_hi, fits ? (float)m_lo : -(float)m_lo, (float)m_hi);
+    if (!fits) rest[t] = 1;
+}
+
+__global__ void px_fill_kernel(int* __restrict__ p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
 }
 
 // Window of a (tile, view) pair: detector columns floor(fu_lo)-1 .. floor(fu_hi)+2 and rows
@@ -95,29 +122,31 @@ struct PxWindow {
     int c0, cols, r0, rows;
 };
 
+// V = m z is smallest at (zlo, m_lo) for zlo >= 0, else at (zlo, m_hi), and largest at
+// (zhi, m_hi) for zhi >= 0, else at (zhi, m_lo); `lo_hi`/`hi_hi` say which m each end takes.
 template <int ROWS>
-__device__ inline PxWindow px_window(const float4 bnd, float zlo, float zhi, const PxArgs& a) {
+__device__ inline PxWindow px_window(const float4 bnd, float zlo_p, float zhi_p, bool lo_hi,
+                                     bool hi_hi, const PxArgs& a) {
     PxWindow w;
     w.c0 = (int)floorf(bnd.x) - 1;
     w.cols = (int)floorf(bnd.y) + 3 - w.c0;
-    const float va = bnd.z * zlo, vb = bnd.w * zlo, vc = bnd.z * zhi, vd = bnd.w * zhi;
-    const float fv_lo = fmaf(fminf(fminf(va, vb), fminf(vc, vd)), a.inv_dp, a.v_mid);
-    const float fv_hi = fmaf(fmaxf(fmaxf(va, vb), fmaxf(vc, vd)), a.inv_dp, a.v_mid);
+    const float fv_lo = fmaf(lo_hi ? bnd.w : bnd.z, zlo_p, a.v_mid);
+    const float fv_hi = fmaf(hi_hi ? bnd.w : bnd.z, zhi_p, a.v_mid);
     w.r0 = (int)floorf(fv_lo) - 1;
     w.rows = (int)floorf(fv_hi) + 3 - w.r0;
+    // the tile pass admits a window only with two cells to spare; the clamps keep every LDS
+    // index inside the tile whatever float32 rounding does
+    w.cols = min(w.cols, kPxPitch - 1);
+    w.rows = min(w.rows, ROWS);
     return w;
-}
-
-template <int ROWS>
-__device__ inline bool px_fits(const PxWindow& w) {
-    return w.cols <= kPxPitch - 1 && w.rows <= ROWS;
 }
 
 // One group per 16 x 16 x TZ voxel tile.  Per view, the window rows are fetched with one buffer
 // load per (row, column) into registers a view ahead, written to one of two LDS buffers and
 // read back by the voxels; a single barrier per view separates the write of a buffer from its
 // reads two views earlier.  Cells outside the detector get a buffer offset past the view and
-// the range check returns 0 for them.
+// the range check returns 0 for them.  Views whose window does not fit (m_lo < 0 in the tile
+// table) are left to pixel_backproject_rest_kernel.
 template <int TZ, int ROWS>
 __global__ void __launch_bounds__(kPxThreads) pixel_backproject_kernel(
     const float* __restrict__ q, float* __restrict__ vol, const float2* __restrict__ cs,
@@ -142,6 +171,8 @@ __global__ void __launch_bounds__(kPxThreads) pixel_backproject_kernel(
     const float y = fmaf((float)min(iy, a.ny - 1), a.dy, a.y_first);
     const float zlo = fmaf((float)iz0, a.dz, a.z_first);
     const float zhi = fmaf((float)(iz0 + TZ - 1), a.dz, a.z_first);
+    const float zlo_p = zlo * a.inv_dp, zhi_p = zhi * a.inv_dp;  // z in detector rows
+    const bool lo_hi = zlo < 0.0f, hi_hi = zhi >= 0.0f;
     const size_t view_cells = (size_t)a.nu * a.nv;
     const int view_bytes = (int)(view_cells * 4);
 
@@ -177,8 +208,9 @@ __global__ void __launch_bounds__(kPxThreads) pixel_backproject_kernel(
         }
     };
 
-    PxWindow wn = px_window<ROWS>(tb[tile], zlo, zhi, a);
-    bool fits_n = px_fits<ROWS>(wn);
+    float4 bn = tb[tile];
+    PxWindow wn = px_window<ROWS>(bn, zlo_p, zhi_p, lo_hi, hi_hi, a);
+    bool fits_n = bn.z > 0.0f;
     if (fits_n) fetch(0, wn);
 
     for (int v = 0; v < a.n_views; ++v) {
@@ -187,11 +219,13 @@ __global__ void __launch_bounds__(kPxThreads) pixel_backproject_kernel(
         float* buf = win + (v & 1) * kBuf;
         if (fits) put(buf, w);
         if (v + 1 < a.n_views) {
-            wn = px_window<ROWS>(tb[(size_t)(v + 1) * a.n_tiles_xy + tile], zlo, zhi, a);
-            fits_n = px_fits<ROWS>(wn);
+            bn = tb[(size_t)(v + 1) * a.n_tiles_xy + tile];
+            wn = px_window<ROWS>(bn, zlo_p, zhi_p, lo_hi, hi_hi, a);
+            fits_n = bn.z > 0.0f;
             if (fits_n) fetch(v + 1, wn);
         }
         __syncthreads();
+        if (!fits) continue;
 
         // this column's sample geometry (python/ConeBeamGPU.py:125-127)
         const float2 sc = cs[v];
@@ -205,49 +239,28 @@ __global__ void __launch_bounds__(kPxThreads) pixel_backproject_kernel(
         // iu = floor(fu) except on the last grid column (fu = nu-1: iu = nu-2, fx = 1)
         const int iu = min((int)floorf(fu), a.nu - 2);
         const float fx = fu - (float)iu;
-
-        if (fits) {
-            // every row floor(fv(k)) must lie in [r0, r0 + rows - 2]; fv is monotone in k
-            const float f_last = fmaf((float)(TZ - 1), kz, fvb);
-            const bool in_win = floorf(fminf(fvb, f_last)) >= (float)w.r0 &&
-                                floorf(fmaxf(fvb, f_last)) <= (float)(w.r0 + w.rows - 2);
-            const float wc = (uok && in_win) ? m * m * a.scale : 0.0f;
-            const float kz_c = in_win ? kz : 0.0f;
-            const float fvb_c = in_win ? fvb : (float)w.r0;
-            const int cu = min(max(iu - w.c0, 0), w.cols - 2);
-            const float* base = buf + cu - w.r0 * kPxPitch;
+        // every row floor(fv(k)) must lie in [r0, r0 + rows - 2]: fv(k) is monotone in k, so
+        // both ends in [r0, r0 + rows - 1) suffice (false for NaN)
+        const float f_last = fmaf((float)(TZ - 1), kz, fvb);
+        const float wr0 = (float)w.r0, wr1 = (float)(w.r0 + w.rows - 1);
+        const bool in_win = fvb >= wr0 && fvb < wr1 && f_last >= wr0 && f_last < wr1;
+        const float wc = (uok && in_win) ? m * m * a.scale : 0.0f;
+        const float kz_c = in_win ? kz : 0.0f;
+        const float fvb_c = in_win ? fvb : wr0;
+        const int cu = min(max(iu - w.c0, 0), w.cols - 2);
+        const float* base = buf + cu - w.r0 * kPxPitch;
 #pragma unroll
-            for (int k = 0; k < TZ; ++k) {
-                const float fv = fmaf((float)k, kz_c, fvb_c);
-                const int iv = (int)floorf(fv);
-                const float fy = __builtin_amdgcn_fractf(fv);
-                const float* p = base + __mul24(iv, kPxPitch);
-                const float q00 = p[0], q01 = p[1], q10 = p[kPxPitch], q11 = p[kPxPitch + 1];
-                const float m0 = fmaf(fx, q01 - q00, q00);
-                const float m1 = fmaf(fx, q11 - q10, q10);
-                const float val = fmaf(fy, m1 - m0, m0);
-                const float wk = (fv >= 0.0f && fv <= a.v_last) ? wc : 0.0f;
-                acc[k] = fmaf(val, wk, acc[k]);
-            }
-        } else {
-            // window larger than the LDS tile: the four samples come from global memory
-            const float* qv = q + (size_t)v * view_cells;
-            const int iuc = max(iu, 0);
-            const float fxc = __saturatef(fu - (float)iuc);
-            const float wc = uok ? m * m * a.scale : 0.0f;
-#pragma unroll
-            for (int k = 0; k < TZ; ++k) {
-                const float fv = fmaf((float)k, kz, fvb);
-                const bool ok = fv >= 0.0f && fv <= a.v_last;
-                const int iv = min(max((int)floorf(fv), 0), a.nv - 2);
-                const float fy = __saturatef(fv - (float)iv);
-                const float* r0p = qv + (size_t)iv * a.nu + iuc;
-                const float q00 = r0p[0], q01 = r0p[1], q10 = r0p[a.nu], q11 = r0p[a.nu + 1];
-                const float m0 = fmaf(fxc, q01 - q00, q00);
-                const float m1 = fmaf(fxc, q11 - q10, q10);
-                const float val = fmaf(fy, m1 - m0, m0);
-                acc[k] = fmaf(val, ok ? wc : 0.0f, acc[k]);
-            }
+        for (int k = 0; k < TZ; ++k) {
+            const float fv = fmaf((float)k, kz_c, fvb_c);
+            const int iv = (int)floorf(fv);
+            const float fy = __builtin_amdgcn_fractf(fv);
+            const float* p = base + __mul24(iv, kPxPitch);
+            const float q00 = p[0], q01 = p[1], q10 = p[kPxPitch], q11 = p[kPxPitch + 1];
+            const float m0 = fmaf(fx, q01 - q00, q00);
+            const float m1 = fmaf(fx, q11 - q10, q10);
+            const float val = fmaf(fy, m1 - m0, m0);
+            const float wk = (fv >= 0.0f && fv <= a.v_last) ? wc : 0.0f;
+            acc[k] = fmaf(val, wk, acc[k]);
         }
     }
 
@@ -263,15 +276,80 @@ __global__ void __launch_bounds__(kPxThreads) pixel_backproject_kernel(
     }
 }
 
+// The views whose window does not fit the LDS tile, for the tiles that have any: the four
+// samples of a voxel come straight from global memory (indices clamped to the detector, the
+// sample weighted 0 off it), and the sums are added to what pixel_backproject_kernel stored.
+template <int TZ>
+__global__ void __launch_bounds__(kPxThreads) pixel_backproject_rest_kernel(
+    const float* __restrict__ q, float* __restrict__ vol, const float2* __restrict__ cs,
+    const float4* __restrict__ tb, const int* __restrict__ rest, PxArgs a) {
+    const int tid = threadIdx.x;
+    const int tx = tid & (kPxTile - 1), ty = tid / kPxTile;
+    int b = blockIdx.x;
+    const int bx = b % a.tiles_x;
+    b /= a.tiles_x;
+    const int by = b % a.tiles_y;
+    const int bz = b / a.tiles_y;
+    const int tile = by * a.tiles_x + bx;
+    if (rest[tile] == 0) return;
+    const int ix = bx * kPxTile + tx, iy = by * kPxTile + ty, iz0 = bz * TZ;
+    const float x = fmaf((float)min(ix, a.nx - 1), a.dx, a.x_first);
+    const float y = fmaf((float)min(iy, a.ny - 1), a.dy, a.y_first);
+    const float zlo = fmaf((float)iz0, a.dz, a.z_first);
+    const size_t view_cells = (size_t)a.nu * a.nv;
+
+    float acc[TZ];
+#pragma unroll
+    for (int k = 0; k < TZ; ++k) acc[k] = 0.0f;
+    for (int v = 0; v < a.n_views; ++v) {
+        if (tb[(size_t)v * a.n_tiles_xy + tile].z > 0.0f) continue;
+        const float2 sc = cs[v];
+        const float t = fmaf(x, sc.x, y * sc.y);
+        const float s = fmaf(y, sc.x, -x * sc.y);
+        const float m = a.R * __builtin_amdgcn_rcpf(a.R - s);
+        const float fu = fmaf(t * m, a.inv_dk, a.u_mid);
+        const bool uok = fu >= 0.0f && fu <= a.u_last;
+        const float kz = m * a.dz * a.inv_dp;
+        const float fvb = fmaf(m * zlo, a.inv_dp, a.v_mid);
+        const float* qv = q + (size_t)v * view_cells;
+        const int iu = min(max((int)floorf(fu), 0), a.nu - 2);
+        const float fx = __saturatef(fu - (float)iu);
+        const float wc = uok ? m * m * a.scale : 0.0f;
+#pragma unroll
+        for (int k = 0; k < TZ; ++k) {
+            const float fv = fmaf((float)k, kz, fvb);
+            const bool ok = fv >= 0.0f && fv <= a.v_last;
+            const int iv = min(max((int)floorf(fv), 0), a.nv - 2);
+            const float fy = __saturatef(fv - (float)iv);
+            const float* r0p = qv + (size_t)iv * a.nu + iu;
+            const float q00 = r0p[0], q01 = r0p[1], q10 = r0p[a.nu], q11 = r0p[a.nu + 1];
+            const float m0 = fmaf(fx, q01 - q00, q00);
+            const float m1 = fmaf(fx, q11 - q10, q10);
+            const float val = fmaf(fy, m1 - m0, m0);
+            acc[k] = fmaf(val, ok ? wc : 0.0f, acc[k]);
+        }
+    }
+    if (ix < a.nx && iy < a.ny) {
+#pragma unroll
+        for (int k = 0; k < TZ; ++k) {
+            const int iz = iz0 + k;
+            if (iz < a.nz) {
+                const size_t o = ((size_t)iz * a.ny + iy) * a.nx + ix;
+                vol[o] += acc[k];
+            }
+        }
+    }
+}
+
 struct PxLayout {
-    size_t cs_off, tb_off, total;
+    size_t cs_off, tb_off, rest_off, total;
     int tiles_x, tiles_y, slabs, tz;
 };
 
 // z voxels per work-item: kPxTz when the window of the outermost tile (largest magnification
-// m_max = R/(R - r_max), its change across a tile, the largest |z|) stays within kPxRows rows,
-// else kPxTzShort.  (Either way a (tile, view) window that does not fit is read from global
-// memory.)
+// m_max = R/(R - r_max), its change across a tile, the largest |z|) stays within the LDS tile
+// with the tile pass's two rows of margin, else kPxTzShort.  (Either way a (tile, view) window
+// that does not fit is read from global memory by the rest pass.)
 int px_choose_tz(const ctr_geometry* g) {
     const double xr = fabs(0.5 * (g->nx - 1) * g->dx) + fabs(g->recon_center[0]);
     const double yr = fabs(0.5 * (g->ny - 1) * g->dy) + fabs(g->recon_center[1]);
@@ -281,7 +359,7 @@ int px_choose_tz(const ctr_geometry* g) {
     const double tile_span = kPxTile * sqrt(g->dx * g->dx + g->dy * g->dy);
     const double rows = (kPxTz - 1) * g->dz * m_max / dp +
                         m_max * m_max * tile_span / g->sad * z_max / dp + 5.0;
-    return rows <= kPxRows ? kPxTz : kPxTzShort;
+    return rows <= kPxRows - 2 ? kPxTz : kPxTzShort;
 }
 
 PxLayout px_layout(const ctr_geometry* g) {
@@ -292,7 +370,8 @@ PxLayout px_layout(const ctr_geometry* g) {
     l.slabs = (g->nz + l.tz - 1) / l.tz;
     l.cs_off = 0;
     l.tb_off = align_up((size_t)g->n_views * sizeof(float2), 256);
-    l.total = l.tb_off + align_up((size_t)g->n_views * l.tiles_x * l.tiles_y * sizeof(float4), 256);
+    l.rest_off = l.tb_off + align_up((size_t)g->n_views * l.tiles_x * l.tiles_y * sizeof(float4), 256);
+    l.total = l.rest_off + align_up((size_t)l.tiles_x * l.tiles_y * sizeof(int), 256);
     return l;
 }
 
@@ -343,6 +422,7 @@ int ctr_pixel_backproject(const ctr_geometry* g, const double* angles, const flo
     char* ws = (char*)workspace;
     float2* cs = (float2*)(ws + l.cs_off);
     float4* tb = (float4*)(ws + l.tb_off);
+    int* rest = (int*)(ws + l.rest_off);
     const double mag = g->sad / g->sdd;
     const double x_first = -0.5 * (g->nx - 1) * g->dx + g->recon_center[0];
     const double y_first = -0.5 * (g->ny - 1) * g->dy + g->recon_center[1];
@@ -350,6 +430,9 @@ int ctr_pixel_backproject(const ctr_geometry* g, const double* angles, const flo
     const double inv_dk = 1.0 / (g->du * mag), inv_dp = 1.0 / (g->dv * mag);
     const int n_tiles_xy = l.tiles_x * l.tiles_y;
 
+    hipLaunchKernelGGL(px_fill_kernel, dim3((n_tiles_xy + 255) / 256), dim3(256), 0, s, rest,
+                       n_tiles_xy);
+    if ((st = status_from_hip(hipGetLastError()))) return st;
     hipLaunchKernelGGL(px_view_kernel, dim3((g->n_views + 255) / 256), dim3(256), 0, s, angles,
                        cs, g->n_views);
     if ((st = status_from_hip(hipGetLastError()))) return st;
@@ -359,16 +442,23 @@ int ctr_pixel_backproject(const ctr_geometry* g, const double* angles, const flo
     ta.n_tiles_xy = n_tiles_xy;
     ta.nx = g->nx;
     ta.ny = g->ny;
-    ta.pad = 0;
+    ta.tz = l.tz;
+    ta.slabs = l.slabs;
+    ta.rows_cap = kPxRows;
     ta.x_first = x_first;
     ta.y_first = y_first;
+    ta.z_first = z_first;
     ta.dx = g->dx;
     ta.dy = g->dy;
+    ta.dz = g->dz;
     ta.R = g->sad;
     ta.inv_dk = inv_dk;
     ta.u_mid = 0.5 * (g->nu - 1);
+    ta.inv_dp = inv_dp;
+    ta.v_mid = 0.5 * (g->nv - 1);
     const int n_tb = g->n_views * n_tiles_xy;
-    hipLaunchKernelGGL(px_tile_kernel, dim3((n_tb + 255) / 256), dim3(256), 0, s, angles, tb, ta);
+    hipLaunchKernelGGL(px_tile_kernel, dim3((n_tb + 255) / 256), dim3(256), 0, s, angles, tb,
+                       rest, ta);
     if ((st = status_from_hip(hipGetLastError()))) return st;
 
     PxArgs a;
@@ -397,11 +487,18 @@ int ctr_pixel_backproject(const ctr_geometry* g, const double* angles, const flo
     a.v_last = (float)(g->nv - 1);
     a.scale = scale;
     const dim3 grid((unsigned)(n_tiles_xy * l.slabs));
-    if (l.tz == kPxTz)
+    if (l.tz == kPxTz) {
         hipLaunchKernelGGL(HIP_KERNEL_NAME(pixel_backproject_kernel<kPxTz, kPxRows>), grid,
                            dim3(kPxThreads), 0, s, q, vol, cs, tb, a);
-    else
+        if ((st = status_from_hip(hipGetLastError()))) return st;
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(pixel_backproject_rest_kernel<kPxTz>), grid,
+                           dim3(kPxThreads), 0, s, q, vol, cs, tb, rest, a);
+    } else {
         hipLaunchKernelGGL(HIP_KERNEL_NAME(pixel_backproject_kernel<kPxTzShort, kPxRows>), grid,
                            dim3(kPxThreads), 0, s, q, vol, cs, tb, a);
+        if ((st = status_from_hip(hipGetLastError()))) return st;
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(pixel_backproject_rest_kernel<kPxTzShort>), grid,
+                           dim3(kPxThreads), 0, s, q, vol, cs, tb, rest, a);
+    }
     return status_from_hip(hipGetLastError());
 }
