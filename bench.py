@@ -225,7 +225,8 @@ def pixel_fdk_leg(dev, stream, steps, warmup, cpu=True, seed=3):
     prof = profile_record("pixel_backproject_traffic", PIXEL_KERNEL)
     out = {"workload": "pixel-driven FDK 512^3, 680 views, 1024x768 detector (python/ReconstructionParams.txt)",
            "s": f_s + b_s, "filter_s": f_s, "backprojection_s": b_s, "gups": updates / b_s / 1e9,
-           "kernel": PIXEL_KERNEL, "valu_busy": prof.get("valu_busy_frac"),
+           "kernel": PIXEL_KERNEL, "bound": "valu issue",
+           "valu_issue_per_simd_cycle": prof.get("valu_issue_per_simd_cycle"),
            "hbm_traffic_bytes": prof.get("traffic_bytes")}
     if cpu:
         from oracle import pixel_oracle as PO
