@@ -1,5 +1,6 @@
 """The geometry text-file loader (python/ConeBeam.py:51-88, python/FanBeam.py:43-75)."""
 import math
+import os
 
 import numpy as np
 import pytest
@@ -8,30 +9,10 @@ from ctreconstruction_amd import paramsfile as PF
 from ctreconstruction_amd.geometry import ScanGeometry
 from oracle import dd_oracle as O
 
-# same keys and values as python/ReconstructionParams.txt:1-11 and ReconstructionParamsFan.txt:1-10
-CONE = """DataPath: ./CBCTData/LogData/
-SourceToAxis: 1000
-SourceToDetector: 1500
-DetectorPixelWidth: 0.390625
-DetectorPixelHeight: 0.390625
-DetectorWidth: 1024
-DetectorHeight: 768
-NumberOfViews: 680
-AngleCoverage: 360
-precision: float32
-ReconVolume: 512*512*512
-"""
-FAN = """DataPath: ./FanBeam/LogData/
-SourceToAxis: 1000
-SourceToDetector: 1500
-DetectorPixelWidth: 0.593412
-DetectorWidth: 750
-DetectorHeight: 200
-NumberOfViews: 680
-AngleCoverage: 360
-precision: float32
-ReconVolume: 512*512
-"""
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+# the reference's geometry files (python/ReconstructionParams.txt:1-11, ReconstructionParamsFan.txt:1-10)
+CONE = open(os.path.join(GOLDEN, "ReconstructionParams.txt")).read()
+FAN = open(os.path.join(GOLDEN, "ReconstructionParamsFan.txt")).read()
 
 
 @pytest.fixture
@@ -113,3 +94,21 @@ def test_legacy_grid_mapping():
         np.testing.assert_allclose(planes, (pc[ax] - n[ax] / 2 + k) * d[ax] - d[ax] / 2, rtol=0, atol=1e-12)
         i = np.arange(n[ax])
         np.testing.assert_allclose(centres, (pc[ax] - n[ax] / 2 + i) * d[ax], rtol=0, atol=1e-12)
+
+
+def test_pixel_scan_of_the_reference_geometry_file():
+    # The GPU ConeBeam's reader (adds Mode) on the shipped file, and the scan derived from it:
+    # fov/2 = R sin(atan(W/2/SDD)) = R (W/2) / sqrt(SDD^2 + (W/2)^2), 680 views over 2 pi.
+    from ctreconstruction_amd.pixel_fdk import PIXEL_KEYS, PixelScan
+    prm = PF.read_params_file(os.path.join(GOLDEN, "ReconstructionParams.txt"), keys=PIXEL_KEYS)
+    assert prm["Mode"] == 0 and prm["precision"] == "float32"
+    assert (prm["ReconX"], prm["ReconY"], prm["ReconZ"]) == (512, 512, 512)
+    scan = PixelScan(prm)
+    half_w, half_h = 1024 * 0.390625 / 2, 768 * 0.390625 / 2
+    assert scan.fov == pytest.approx(2 * 1000 * half_w / math.hypot(1500, half_w), rel=1e-14)
+    assert scan.fovz == pytest.approx(2 * 1000 * half_h / math.hypot(1500, half_h), rel=1e-14)
+    assert scan.dtheta == pytest.approx(2 * math.pi / 680, rel=1e-14)
+    assert scan.zero_padded_length == 2048
+    assert scan.tau == pytest.approx(0.390625 * 1000 / 1500, rel=1e-14)
+    g = scan.ctypes_struct()
+    assert g.dx == pytest.approx(scan.fov / 511) and g.dz == pytest.approx(scan.fovz / 511)
