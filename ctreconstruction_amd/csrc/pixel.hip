@@ -70,10 +70,12 @@ __global__ void px_view_kernel(const double* __restrict__ angles, float2* __rest
 // at the last voxel: the work-items past it sample at that voxel's position.
 //
 // The same pass decides, in FP64 and once for both backprojection kernels, whether the
-// (tile, view) window fits the LDS tile for every z slab: the row span m z / dp over a slab
-// grows with |z|, so the first and the last slab bound it.  Two cells of margin in each
-// direction cover the float32 recomputation of the window in the main kernel.  A window that
-// does not fit is marked by a negative m_lo, and its tile by rest[tile] = 1.
+// (tile, view) window fits the LDS tile for every z slab.  The main kernel sizes a window as
+// floor(hi) - floor(lo) + 4 cells from float32 ends hi, lo; since floor(a) - floor(b) <=
+// ceil(a - b) and float32 moves each end by far less than half a cell, that is at most
+// ceil(span) + 5 for the FP64 span hi - lo.  The row span m z / dp of a slab grows with |z|, so
+// the first and the last slab bound it.  A window that does not fit is marked by a negative m_lo,
+// and its tile by rest[tile] = 1.
 __global__ void px_tile_kernel(const double* __restrict__ angles, float4* __restrict__ tb,
                                int* __restrict__ rest, PxTileArgs a) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
@@ -99,16 +101,16 @@ __global__ void px_tile_kernel(const double* __restrict__ angles, float4* __rest
             m_lo = fmin(m_lo, m);
             m_hi = fmax(m_hi, m);
         }
-    const double cols = floor(fu_hi) - floor(fu_lo) + 4.0;
+    const double cols = ceil(fu_hi - fu_lo) + 5.0;
     double rows = 0.0;
     for (int e = 0; e < 2; ++e) {
         const double zlo = a.z_first + (double)((e ? a.slabs - 1 : 0) * a.tz) * a.dz;
         const double zhi = zlo + (double)(a.tz - 1) * a.dz;
-        const double lo = (zlo < 0.0 ? m_hi : m_lo) * zlo * a.inv_dp + a.v_mid;
-        const double hi = (zhi >= 0.0 ? m_hi : m_lo) * zhi * a.inv_dp + a.v_mid;
-        rows = fmax(rows, floor(hi) - floor(lo) + 4.0);
+        const double lo = (zlo < 0.0 ? m_hi : m_lo) * zlo * a.inv_dp;
+        const double hi = (zhi >= 0.0 ? m_hi : m_lo) * zhi * a.inv_dp;
+        rows = fmax(rows, ceil(hi - lo) + 5.0);
     }
-    const bool fits = cols <= (double)(kPxCols - 2) && rows <= (double)(a.rows_cap - 2);
+    const bool fits = cols <= (double)kPxCols && rows <= (double)a.rows_cap;
     tb[idx] = make_float4((float)fu_lo, (float)fu_hi, fits ? (float)m_lo : -(float)m_lo, (float)m_hi);
     if (!fits) rest[t] = 1;
 }
@@ -136,8 +138,8 @@ __device__ inline PxWindow px_window(const float4 bnd, float zlo_p, float zhi_p,
     const float fv_hi = fmaf(hi_hi ? bnd.w : bnd.z, zhi_p, a.v_mid);
     w.r0 = (int)floorf(fv_lo) - 1;
     w.rows = (int)floorf(fv_hi) + 3 - w.r0;
-    // the tile pass admits a window only with two cells to spare; the clamps keep every LDS
-    // index inside the tile whatever float32 rounding does
+    // the tile pass admits only windows whose float32 size is within the LDS tile; the clamps
+    // keep every LDS index inside it regardless
     w.cols = min(w.cols, kPxCols);
     w.rows = min(w.rows, ROWS);
     return w;
@@ -349,10 +351,10 @@ struct PxLayout {
     int tiles_x, tiles_y, slabs, tz;
 };
 
-// z voxels per work-item: kPxTz when the window of the outermost tile (largest magnification
-// m_max = R/(R - r_max), its change across a tile, the largest |z|) stays within the LDS tile
-// with the tile pass's two rows of margin, else kPxTzShort.  (Either way a (tile, view) window
-// that does not fit is read from global memory by the rest pass.)
+// z voxels per work-item: kPxTz when the row span of the outermost tile (largest magnification
+// m_max = R/(R - r_max), its change across a tile, the largest |z|) passes the tile pass's
+// test ceil(span) + 5 <= kPxRows, else kPxTzShort.  (Either way a (tile, view) window that does
+// not fit is read from global memory by the rest pass.)
 int px_choose_tz(const ctr_geometry* g) {
     const double xr = fabs(0.5 * (g->nx - 1) * g->dx) + fabs(g->recon_center[0]);
     const double yr = fabs(0.5 * (g->ny - 1) * g->dy) + fabs(g->recon_center[1]);
@@ -360,9 +362,9 @@ int px_choose_tz(const ctr_geometry* g) {
     const double dp = g->dv * g->sad / g->sdd;
     const double z_max = fabs(0.5 * (g->nz - 1) * g->dz) + fabs(g->recon_center[2]);
     const double tile_span = kPxTile * sqrt(g->dx * g->dx + g->dy * g->dy);
-    const double rows = (kPxTz - 1) * g->dz * m_max / dp +
-                        m_max * m_max * tile_span / g->sad * z_max / dp + 5.0;
-    return rows <= kPxRows - 2 ? kPxTz : kPxTzShort;
+    const double span = (kPxTz - 1) * g->dz * m_max / dp +
+                        m_max * m_max * tile_span / g->sad * z_max / dp;
+    return ceil(span) + 5.0 <= kPxRows ? kPxTz : kPxTzShort;
 }
 
 PxLayout px_layout(const ctr_geometry* g) {
