@@ -24,7 +24,7 @@ import warnings
 import numpy as np
 import torch
 
-from . import _native as N
+from . import _native as _nat
 from .paramsfile import CONE_KEYS, ParamsFileError, field_of_view, read_params_file
 from .projector import _ptr, _require, _stream, _workspace
 
@@ -76,11 +76,11 @@ class PixelScan:
             return 1.0, -extent / 2.0
         return extent / (n - 1), 0.0
 
-    def ctypes_struct(self) -> N.CtrGeometry:
-        g = N.CtrGeometry()
+    def ctypes_struct(self) -> _nat.CtrGeometry:
+        g = _nat.CtrGeometry()
         g.nx, g.ny, g.nz = self.nx, self.ny, self.nz
         g.nu, g.nv, g.n_views = self.nu, self.nv, self.n_views
-        g.detector_shape = N.CTR_DETECTOR_FLAT
+        g.detector_shape = _nat.CTR_DETECTOR_FLAT
         (g.dx, cx), (g.dy, cy), (g.dz, cz) = (self._axis(self.nx, self.fov), self._axis(self.ny, self.fov),
                                               self._axis(self.nz, self.fovz))
         g.du, g.dv = self.du, self.dv
@@ -95,13 +95,13 @@ class PixelScan:
 def pixel_spectrum(n_fft: int, tau: float, filter_type: str = DEFAULT_FILTER,
                    cutoff: float = DEFAULT_CUTOFF, device="cuda") -> torch.Tensor:
     """``ConeBeam.Filter`` (python/ConeBeamGPU.py:355-384) in FFT order, float64[n_fft], on the GPU."""
-    if filter_type not in N.FILTER_CODES:
-        raise ValueError(f"unknown FilterType {filter_type!r}; expected one of {sorted(N.FILTER_CODES)}")
+    if filter_type not in _nat.FILTER_CODES:
+        raise ValueError(f"unknown FilterType {filter_type!r}; expected one of {sorted(_nat.FILTER_CODES)}")
     if cutoff > 0.5 or cutoff < 0:
         # the reference prints ErrorDescription(4) and carries on (python/ConeBeamGPU.py:357-361)
         warnings.warn("Cutoff have to be pose between 0 and 0.5", stacklevel=2)
     out = torch.empty(int(n_fft), dtype=torch.float64, device=device)
-    N.check(N.lib().ctr_pixel_filter_spectrum(int(n_fft), float(tau), N.FILTER_CODES[filter_type],
+    _nat.check(_nat.lib().ctr_pixel_filter_spectrum(int(n_fft), float(tau), _nat.FILTER_CODES[filter_type],
                                               float(cutoff), _ptr(out), _stream(out.device)),
             "ctr_pixel_filter_spectrum")
     return out
@@ -125,11 +125,11 @@ def pixel_filter(proj: torch.Tensor, scan: PixelScan, spectrum: torch.Tensor | N
     _require(out, tuple(proj.shape), "out")
     if out.data_ptr() == proj.data_ptr():
         raise ValueError("pixel_filter writes mirrored rows and cannot run in place")
-    lib = N.lib()
+    lib = _nat.lib()
     g = scan.ctypes_struct()
     nbytes = lib.ctr_fdk_filter_workspace_bytes(ctypes.byref(g))
     ws = _workspace(nbytes, dev)
-    N.check(lib.ctr_pixel_filter(ctypes.byref(g), _ptr(spectrum), _ptr(proj), _ptr(out),
+    _nat.check(lib.ctr_pixel_filter(ctypes.byref(g), _ptr(spectrum), _ptr(proj), _ptr(out),
                                  int(proj.shape[0]), _ptr(ws), nbytes, _stream(dev)), "ctr_pixel_filter")
     return out
 
@@ -148,11 +148,11 @@ def pixel_backproject(q: torch.Tensor, scan: PixelScan, scale: float | None = No
     if angles is None:
         angles = scan.device_angles(dev)
     _require(angles, (scan.n_views,), "angles", torch.float64)
-    lib = N.lib()
+    lib = _nat.lib()
     g = scan.ctypes_struct()
     nbytes = lib.ctr_pixel_backproject_workspace_bytes(ctypes.byref(g))
     ws = _workspace(nbytes, dev)
-    N.check(lib.ctr_pixel_backproject(ctypes.byref(g), _ptr(angles), _ptr(q), _ptr(out),
+    _nat.check(lib.ctr_pixel_backproject(ctypes.byref(g), _ptr(angles), _ptr(q), _ptr(out),
                                       float(scan.dtheta if scale is None else scale),
                                       int(bool(accumulate)), _ptr(ws), nbytes, _stream(dev)),
             "ctr_pixel_backproject")
@@ -211,7 +211,7 @@ class ConeBeam:
         return self.recon
 
     @staticmethod
-    def Filter(N_: int, pixel_size: float, FilterType: str, cutoff: float, device="cuda") -> np.ndarray:
+    def Filter(N: int, pixel_size: float, FilterType: str, cutoff: float, device="cuda") -> np.ndarray:
         """``ConeBeam.Filter``: the centred magnitude of length N-1, computed on the GPU."""
-        h = pixel_spectrum(int(N_) - 1, pixel_size, FilterType, cutoff, device)
+        h = pixel_spectrum(int(N) - 1, pixel_size, FilterType, cutoff, device)
         return np.fft.fftshift(h.cpu().numpy())

@@ -124,9 +124,11 @@ def test_backproject_matches_oracle(dev, name):
     assert err <= 1e-4 * np.abs(ref).max()
 
 
-def test_backproject_accumulates_and_scales(dev):
+@pytest.mark.parametrize("name", ["ragged", "wide_fan"])
+def test_backproject_accumulates_and_scales(dev, name):
+    # "wide_fan" sends part of its views through the rest pass, which adds to the stored volume
     from ctreconstruction_amd.pixel_fdk import pixel_backproject
-    prm, scan, og = _scan("ragged")
+    prm, scan, og = _scan(name)
     rng = np.random.default_rng(8)
     q = torch.from_numpy(rng.standard_normal(scan.proj_shape).astype(np.float32)).to(dev)
     base = torch.from_numpy(rng.standard_normal(scan.vol_shape).astype(np.float32)).to(dev)

@@ -112,3 +112,29 @@ def test_pixel_scan_of_the_reference_geometry_file():
     assert scan.tau == pytest.approx(0.390625 * 1000 / 1500, rel=1e-14)
     g = scan.ctypes_struct()
     assert g.dx == pytest.approx(scan.fov / 511) and g.dz == pytest.approx(scan.fovz / 511)
+
+
+def test_conebeam_loaddata_errors_follow_the_reference(tmp_path):
+    # python/ConeBeamGPU.py:322-346: unknown precision -> ErrorDescription(2), wrong file count
+    # -> ErrorDescription(3); here both raise ParamsFileError with the reference's message.
+    from ctreconstruction_amd.pixel_fdk import ConeBeam
+    data = tmp_path / "views"
+    data.mkdir()
+    cfg = tmp_path / "g.txt"
+    text = (f"DataPath: {data}/\nSourceToAxis: 100\nSourceToDetector: 150\nDetectorPixelWidth: 0.8\n"
+            "DetectorPixelHeight: 0.8\nDetectorWidth: 8\nDetectorHeight: 6\nNumberOfViews: 3\n"
+            "AngleCoverage: 360\nprecision: {prec}\nReconVolume: 4*4*4\n")
+    cfg.write_text(text.format(prec="float16"))
+    cb = ConeBeam(str(cfg), device="cpu")
+    assert cb.proj.shape == (6, 8, 3) and cb.recon.shape == (4, 4, 4)
+    with pytest.raises(PF.ParamsFileError, match="Unknown data precision"):
+        cb.LoadData()
+    cfg.write_text(text.format(prec="int32"))
+    cb = ConeBeam(str(cfg), device="cpu")
+    np.arange(48, dtype=np.int32).tofile(str(data / "a.dat"))
+    with pytest.raises(PF.ParamsFileError, match="Number of file"):
+        cb.LoadData()
+    for n in ("b", "c"):
+        np.arange(48, dtype=np.int32).tofile(str(data / f"{n}.dat"))
+    cb.LoadData()
+    np.testing.assert_array_equal(cb.proj[:, :, 1], np.arange(48).reshape(6, 8))
