@@ -825,10 +825,16 @@ __device__ inline bool bp_column_view(const BpView& v, const float* __restrict__
         };
         if (KIND == kBpFast || lin) {
             const float alo = Alo - fks, ahi = Ahi - fks;
+            // 1/(hi - lo) of voxel i by one Newton step from voxel i-1's: the footprint height is
+            // affine in i and changes by a small fraction d per voxel (the magnification spread
+            // over one voxel), so the step leaves a relative error of about d^2 (below 1e-8 at
+            // 512^3) and costs two FMAs instead of a quarter-rate reciprocal
+            float rinv = __builtin_amdgcn_rcpf(ahi - alo);
 #pragma unroll
             for (int i = 0; i < ZC; ++i) {
                 const float lo = fmaf((float)i, Dlo, alo), hi = fmaf((float)i, Dhi, ahi);
-                acc[i] = fmaf(integral(hi) - integral(lo), __builtin_amdgcn_rcpf(hi - lo), acc[i]);
+                if (i > 0) rinv *= fmaf(lo - hi, rinv, 2.0f);
+                acc[i] = fmaf(integral(hi) - integral(lo), rinv, acc[i]);
             }
         } else {
 #pragma unroll
