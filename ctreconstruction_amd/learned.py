@@ -10,7 +10,7 @@ mu_water = 0.02/mm, 384 x 384 flat detector at 1.0 mm, SID 1000, SDD 1500, Poiss
 The network (``prior.ResidualPrior``) sees 3 adjacent slices of the input and predicts the
 target's centre slice.  At inference the learned reconstruction is: 20 SART iterations, the
 network on every slice, then 5 SART data-consistency iterations starting from its output.
-Plain SART is the fallback and the judge: if the learned result fits the measured projections
+Plain SART is the fallback and the arbiter: if the learned result fits the measured projections
 worse than the SART iterate it started from (projection-data residual), plain SART continues
 instead.  Reconstructions are in mm^-1; the network works in units of mu_water (x 50).
 """

@@ -42,7 +42,7 @@ def test_learned_beats_sart_at_equal_iterations(setup):
     print(f"learned rmse {e_l:.3e} vs SART-25 {e_25:.3e}; residuals {info}")
     assert not info["fallback"]
     assert e_l < 0.8 * e_25
-    # the judge: the learned result fits the measured data at least as well as its warm start
+    # the arbiter: the learned result fits the measured data at least as well as its warm start
     assert info["residual_learned"] <= info["residual_warm"]
 
 
