@@ -14,8 +14,8 @@
 //   px_fill_kernel               zeroes the per-tile "has views for the rest pass" flags
 //   pixel_backproject_kernel     one workgroup per 16 x 16 x TZ voxel tile, one work-item per
 //                                (x, y) column holding TZ accumulators; per view the tile's
-//                                detector window is staged in LDS (double-buffered, fetched a view
-//                                ahead) and every voxel reads its four samples there (two
+//                                detector window is fetched a view ahead into registers, staged
+//                                in LDS and every voxel reads its four samples there (two
 //                                ds_read2)
 //   pixel_backproject_rest_kernel   the (tile, view) pairs whose window is larger than the LDS
 //                                tile, read straight from global memory and added to the volume
@@ -33,11 +33,11 @@ constexpr int kPxTz = 32;       // z voxels per work-item (kPxTzShort where wind
 constexpr int kPxTzShort = 16;
 // LDS window row pitch (floats): a window has at most 64 columns; 71 (7 mod 32) spreads the
 // samples of an 8 x 8 voxel patch, which lie on a few neighbouring rows, over the banks best
-// among the pitches that keep four groups per CU (measured: 71 -> 35-36 ms, 65 -> 38 ms at
-// 512^3 x 680 views)
+// among the pitches measured (71 -> 35-36 ms, 65 -> 38 ms at 512^3 x 680 views with two windows
+// per group)
 constexpr int kPxPitch = 71;
 constexpr int kPxCols = 64;     // window columns: one per lane of the staging loads
-constexpr int kPxRows = 72;     // LDS window rows (two windows per group: 40 KB, 4 groups per CU)
+constexpr int kPxRows = 72;     // LDS window rows (20 KB per group)
 constexpr int kPxThreads = kPxTile * kPxTile;
 
 size_t align_up(size_t n, size_t a) { return (n + a - 1) / a * a; }
@@ -146,10 +146,11 @@ __device__ inline PxWindow px_window(const float4 bnd, float zlo_p, float zhi_p,
 }
 
 // One group per 16 x 16 x TZ voxel tile.  Per view, the window rows are fetched with one buffer
-// load per (row, column) into registers a view ahead, written to one of two LDS buffers and
-// read back by the voxels; a single barrier per view separates the write of a buffer from its
-// reads two views earlier.  Cells outside the detector get a buffer offset past the view and
-// the range check returns 0 for them.  Views whose window does not fit (m_lo < 0 in the tile
+// load per (row, column) into registers a view ahead, written to the group's LDS window after a
+// barrier (the previous view's reads are done) and read back by the voxels after a second one.
+// One window per group rather than two lets six groups share a CU (the registers allow six,
+// two 20 KB windows only four): 33.3 ms instead of 35.2 at 512^3 x 680 views.  Cells outside the
+// detector get a buffer offset past the view and the range check returns 0 for them.  Views whose window does not fit (m_lo < 0 in the tile
 // table) are left to pixel_backproject_rest_kernel.
 template <int TZ, int ROWS>
 __global__ void __launch_bounds__(kPxThreads) pixel_backproject_kernel(
@@ -157,7 +158,7 @@ __global__ void __launch_bounds__(kPxThreads) pixel_backproject_kernel(
     const float4* __restrict__ tb, PxArgs a) {
     constexpr int kBuf = ROWS * kPxPitch;
     constexpr int kStage = ROWS / 4;
-    __shared__ float win[2 * kBuf];
+    __shared__ float win[kBuf];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -221,16 +222,19 @@ __global__ void __launch_bounds__(kPxThreads) pixel_backproject_kernel(
     for (int v = 0; v < a.n_views; ++v) {
         const PxWindow w = wn;
         const bool fits = fits_n;
-        float* buf = win + (v & 1) * kBuf;
-        if (fits) put(buf, w);
+        float* buf = win;
+        if (fits) {
+            __syncthreads();
+            put(buf, w);
+        }
         if (v + 1 < a.n_views) {
             bn = tb[(size_t)(v + 1) * a.n_tiles_xy + tile];
             wn = px_window<ROWS>(bn, zlo_p, zhi_p, lo_hi, hi_hi, a);
             fits_n = bn.z > 0.0f;
             if (fits_n) fetch(v + 1, wn);
         }
-        __syncthreads();
         if (!fits) continue;
+        __syncthreads();
 
         // this column's sample geometry (python/ConeBeamGPU.py:125-127)
         const float2 sc = cs[v];
