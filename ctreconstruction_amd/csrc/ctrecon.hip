@@ -794,12 +794,24 @@ __device__ inline bool bp_column_view(const BpView& v, const float* __restrict__
             // the three taps are consecutive window cells (weight 0 past the footprint; the
             // window buffers are zero-initialised and padded, so such reads stay finite)
             float* sp = scan;
-            for (int j = 0; j < nr; ++j) {
+            // two rows per step: both rows' loads go out before one wait, and the loop's own
+            // work is shared; an odd last row follows
+            int j = 0;
+            for (; j + 1 < nr; j += 2) {
+                const float* rq = rp + stride;
+                const float rk0 = fmaf(wu[0], rp[0], fmaf(wu[1], rp[1], wu[2] * rp[2]));
+                const float rk1 = fmaf(wu[0], rq[0], fmaf(wu[1], rq[1], wu[2] * rq[2]));
+                csum = fmaf(rk0, W, csum);
+                sp[256] = csum;
+                csum = fmaf(rk1, W, csum);
+                sp[512] = csum;
+                sp += 512;
+                rp += 2 * stride;
+            }
+            if (j < nr) {
                 const float rk = fmaf(wu[0], rp[0], fmaf(wu[1], rp[1], wu[2] * rp[2]));
                 csum = fmaf(rk, W, csum);
-                sp += 256;
-                *sp = csum;
-                rp += stride;
+                sp[256] = csum;
             }
         } else {
             for (int j = 0; j < nr; ++j) {
