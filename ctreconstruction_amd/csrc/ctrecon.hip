@@ -702,20 +702,18 @@ __device__ inline bool bp_column_view(const BpView& v, const float* __restrict__
                                       bool window_in_lds, float* __restrict__ scan, float (&acc)[ZC]) {
     const float xr0 = X * v.c + Y * v.s;
     const float yr0 = Y * v.c - X * v.s;
-    float smin = 3.0e38f, smax = -3.0e38f, rmin = 3.0e38f, rmax = -3.0e38f;
-#pragma unroll
-    for (int corner = 0; corner < 4; ++corner) {
-        const float sx = (corner & 1) ? -hx : hx;
-        const float sy = (corner & 2) ? -hy : hy;
-        const float xr = xr0 + sx * v.c + sy * v.s;
-        const float yr = yr0 + sy * v.c - sx * v.s;
-        const float r = __builtin_amdgcn_rcpf(v.Sy - yr);
-        const float su = (v.Sx - xr) * r;
-        smin = fminf(smin, su);
-        smax = fmaxf(smax, su);
-        rmin = fminf(rmin, r);
-        rmax = fmaxf(rmax, r);
-    }
+    // the corners (+-hx, +-hy) rotate to (xr0, yr0) +- (P, M) and (xr0, yr0) +- (Q, -N)
+    const float P = fmaf(hx, v.c, hy * v.s), Q = fmaf(hx, v.c, -hy * v.s);
+    const float M = fmaf(hy, v.c, -hx * v.s), N = fmaf(hy, v.c, hx * v.s);
+    // corner first, then its distance from the source: one rounding at the source's scale
+    const float r0 = __builtin_amdgcn_rcpf(v.Sy - (yr0 + M)), r1 = __builtin_amdgcn_rcpf(v.Sy - (yr0 - M));
+    const float r2 = __builtin_amdgcn_rcpf(v.Sy - (yr0 - N)), r3 = __builtin_amdgcn_rcpf(v.Sy - (yr0 + N));
+    const float su0 = (v.Sx - (xr0 + P)) * r0, su1 = (v.Sx - (xr0 - P)) * r1;
+    const float su2 = (v.Sx - (xr0 + Q)) * r2, su3 = (v.Sx - (xr0 - Q)) * r3;
+    const float smin = fminf(fminf(su0, su1), fminf(su2, su3));
+    const float smax = fmaxf(fmaxf(su0, su1), fmaxf(su2, su3));
+    const float rmin = fminf(fminf(r0, r1), fminf(r2, r3));
+    const float rmax = fmaxf(fmaxf(r0, r1), fmaxf(r2, r3));
     const float fu1 = u_index(v, smin, curved), fu2 = u_index(v, smax, curved);
     const float ulo = fminf(fu1, fu2), uhi = fmaxf(fu1, fu2);
     const int ku0 = (int)floorf(ulo), ku1 = (int)floorf(uhi);
@@ -789,7 +787,8 @@ __device__ inline bool bp_column_view(const BpView& v, const float* __restrict__
         // copy of the total at ks+nr+1 so that a lookup at the top needs no index clamp
         float csum = 0.0f;
         scan[0] = 0.0f;
-        const float* rp = src + (size_t)(ks - row0) * stride + cbase;
+        // window offsets fit 24 bits: a full-rate multiply (the 32-bit one is quarter rate)
+        const float* rp = src + (FROM_LDS ? __mul24(ks - row0, stride) : (size_t)(ks - row0) * stride) + cbase;
         if (KIND == kBpFast || (FROM_LDS && !__any(ntaps > kMaxUTaps))) {
             // the three taps are consecutive window cells (weight 0 past the footprint; the
             // window buffers are zero-initialised and padded, so such reads stay finite)
