@@ -49,9 +49,10 @@ int check_geometry(const ctr_geometry* g) {
     const double cells = (double)g->n_views * g->nv * g->nu;
     if (vox >= 2147483647.0 || cells >= 2147483647.0) return CTR_ERR_INVALID;
     // grid y/z limits: z-prefix tables (ny), backprojection/adjoint (ny/4, nz/4); the
-    // forward projector packs transverse cell indices into 24 bits
+    // forward projector packs transverse cell indices into 24 bits, the backprojector
+    // addresses window rows with 24-bit multiplies by nu
     if (g->n_views > 65535 || g->nv > 65535 * 4 || g->ny > 65535 || g->nz > 65535 ||
-        g->nx >= (1 << 24))
+        g->nx >= (1 << 24) || g->nu >= (1 << 23))
         return CTR_ERR_INVALID;
     return CTR_OK;
 }
@@ -948,9 +949,13 @@ __global__ void __launch_bounds__(256) dd_backproject_tile_kernel(
             wring[view & (kWinRing - 1)] = w;
         }
     };
+    // every thread reads the same ring entry: the window is wave-uniform (scalar registers)
     auto window_of = [&](int view, int& v0, int& vn, int& u0, int& un) {
         const int4 w = wring[view & (kWinRing - 1)];
-        v0 = w.x; vn = w.y; u0 = w.z; un = w.w;
+        v0 = __builtin_amdgcn_readfirstlane(w.x);
+        vn = __builtin_amdgcn_readfirstlane(w.y);
+        u0 = __builtin_amdgcn_readfirstlane(w.z);
+        un = __builtin_amdgcn_readfirstlane(w.w);
     };
     fill_windows(0, kWinRing);
     __syncthreads();
@@ -960,13 +965,16 @@ __global__ void __launch_bounds__(256) dd_backproject_tile_kernel(
             if (view >= n_views || !tflags[view]) return;
         }
         const float* qn = q + (size_t)view * nv * nu + (size_t)v0 * nu + u0;
-        // element e = threadIdx.x + 256 p of the window is (row r, column c); step by 256
-        int r = (int)threadIdx.x / un;
-        int c = (int)threadIdx.x - r * un;
-        const int dr = 256 / un, dc = 256 - dr * un;
+        // element e = threadIdx.x + 256 p of the window is (row r, column c); step by 256.
+        // floor((e + 1/2) / un) in FP32: the quotient is at least 0.5/un from an integer and
+        // its rounding error below 3e-5/un, so the floor is exact (e <= 256, un <= kWinCap)
+        const float inv_un = __builtin_amdgcn_rcpf((float)un);
+        int r = (int)(((float)threadIdx.x + 0.5f) * inv_un);
+        int c = (int)threadIdx.x - __mul24(r, un);
+        const int dr = (int)(256.5f * inv_un), dc = 256 - __mul24(dr, un);
 #pragma unroll
         for (int p = 0; p < kPrefetch; ++p) {
-            if (r < vn) pf[p] = qn[(size_t)r * nu + c];
+            if (r < vn) pf[p] = qn[__mul24(r, nu) + c];
             r += dr;
             c += dc;
             if (c >= un) {
