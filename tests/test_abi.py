@@ -90,6 +90,11 @@ def test_invalid_geometry_is_rejected_before_launch(lib):
     g.dz = -0.5
     assert lib.ctr_dd_backproject(ctypes.byref(g), dummy, dummy, dummy, dummy, 0, 1.0, 0.0, 0,
                                   dummy, 1 << 30, None) == N.CTR_ERR_INVALID
+    # window rows are addressed with 24-bit multiplies by nu
+    g = geom.ctypes_struct()
+    g.nu, g.nv, g.n_views = 1 << 23, 1, 1
+    assert lib.ctr_dd_backproject(ctypes.byref(g), dummy, dummy, dummy, dummy, 0, 1.0, 0.0, 0,
+                                  dummy, 1 << 30, None) == N.CTR_ERR_INVALID
     g = geom.ctypes_struct()
     g.detector_shape = 7
     assert lib.ctr_fdk_filter(ctypes.byref(g), dummy, dummy, dummy, 1, dummy, 1 << 30, None) == \
