@@ -463,11 +463,12 @@ __global__ void __launch_bounds__(256) dd_forward_column_kernel(
                 const int kseg = kd & 0xffff, da = (kd >> 16) & 3, nq16 = (kd >> 18) << 4;
                 const int base = __builtin_amdgcn_readlane(b_o0, q) + 4 * kseg;
                 // quads past the breakpoints in use get an offset beyond the buffer: the range
-                // check returns 0 for them without a memory access
+                // check returns 0 for them without a memory access.  The line index is at most 2
+                // and a line below 2^23 bytes (nz <= 65535): a full-rate 24-bit multiply
 #pragma unroll
                 for (int i = 0; i < NL; ++i)
                     S.d[i] = __builtin_amdgcn_raw_buffer_load_b128(
-                        rsrc, seg_j16[NL - 1][i] < nq16 ? min(seg_l[NL - 1][i], da) * line_bytes + seg_j16[NL - 1][i]
+                        rsrc, seg_j16[NL - 1][i] < nq16 ? __mul24(min(seg_l[NL - 1][i], da), line_bytes) + seg_j16[NL - 1][i]
                                                         : 0x7ffffff0, base, 0);
                 S.rel = off - 4u * (unsigned)kseg;
                 S.w[0] = lane_f(b_w0, q);
