@@ -841,7 +841,7 @@ __device__ inline bool bp_column_view(const BpView& v, const float* __restrict__
             // 1/(hi - lo) of voxel i by one Newton step from voxel i-1's: the footprint height is
             // affine in i and changes by a small fraction d per voxel (the magnification spread
             // over one voxel), so the step leaves a relative error of about d^2 (below 1e-8 at
-            // 512^3) and costs two FMAs instead of a quarter-rate reciprocal
+            // 512^3) and costs two FMAs instead of a reciprocal, which issues at half the FMA rate
             float rinv = __builtin_amdgcn_rcpf(ahi - alo);
 #pragma unroll
             for (int i = 0; i < ZC; ++i) {
