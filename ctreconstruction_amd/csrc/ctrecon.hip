@@ -642,9 +642,12 @@ __device__ inline BpView load_view(const float* __restrict__ vt, int view) {
     return v;
 }
 
-// u index coordinate of the ray through the source with in-plane slope `su`
+// u index coordinate of the ray through the source with in-plane slope `su`.  The footprint
+// arithmetic below is written with explicit fmaf (nothing left for the compiler to contract):
+// the fast and the completing pass decide from the same values which (work-item, view) pairs
+// are the common case, so both must round them identically.
 __device__ inline float u_index(const BpView& v, float su, int curved) {
-    return curved ? v.Au - v.Bu * atanf(su) : v.Au - v.Bu * su;
+    return fmaf(-v.Bu, curved ? atanf(su) : su, v.Au);
 }
 
 // Detector window (first row, rows, first column, columns) covering the projection of the box
@@ -658,8 +661,8 @@ __device__ inline void tile_window(const BpView& v, float xa, float xb, float ya
     for (int corner = 0; corner < 4; ++corner) {
         const float x = (corner & 1) ? xb : xa;
         const float y = (corner & 2) ? yb : ya;
-        const float xr = x * v.c + y * v.s;
-        const float yr = y * v.c - x * v.s;
+        const float xr = fmaf(x, v.c, y * v.s);
+        const float yr = fmaf(y, v.c, -(x * v.s));
         const float r = __builtin_amdgcn_rcpf(v.Sy - yr);
         const float su = (v.Sx - xr) * r;
         smin = fminf(smin, su);
@@ -673,7 +676,7 @@ __device__ inline void tile_window(const BpView& v, float xa, float xb, float ya
     const float ehi = v.Sz - za, elo = v.Sz - zb;  // za <= zb
     const float smx = ehi * (ehi >= 0.0f ? rmax : rmin);
     const float smn = elo * (elo >= 0.0f ? rmin : rmax);
-    const float g1 = v.Av - v.Bv * smn, g2 = v.Av - v.Bv * smx;
+    const float g1 = fmaf(-v.Bv, smn, v.Av), g2 = fmaf(-v.Bv, smx, v.Av);
     const int v0 = max((int)floorf(fminf(g1, g2)) - 1, 0);
     const int v1 = min((int)floorf(fmaxf(g1, g2)) + 1, nv - 1);
     wu0 = u0;
@@ -702,11 +705,11 @@ __device__ inline bool bp_column_view(const BpView& v, const float* __restrict__
                                       float Z0, float dz, float hx, float hy, float hz, int nz_left,
                                       int curved, float scale, float zero_tol, int nu, int nv,
                                       bool window_in_lds, float* __restrict__ scan, float (&acc)[ZC]) {
-    const float xr0 = X * v.c + Y * v.s;
-    const float yr0 = Y * v.c - X * v.s;
+    const float xr0 = fmaf(X, v.c, Y * v.s);
+    const float yr0 = fmaf(Y, v.c, -(X * v.s));
     // the corners (+-hx, +-hy) rotate to (xr0, yr0) +- (P, M) and (xr0, yr0) +- (Q, -N)
-    const float P = fmaf(hx, v.c, hy * v.s), Q = fmaf(hx, v.c, -hy * v.s);
-    const float M = fmaf(hy, v.c, -hx * v.s), N = fmaf(hy, v.c, hx * v.s);
+    const float P = fmaf(hx, v.c, hy * v.s), Q = fmaf(hx, v.c, -(hy * v.s));
+    const float M = fmaf(hy, v.c, -(hx * v.s)), N = fmaf(hy, v.c, hx * v.s);
     // corner first, then its distance from the source: one rounding at the source's scale
     const float r0 = __builtin_amdgcn_rcpf(v.Sy - (yr0 + M)), r1 = __builtin_amdgcn_rcpf(v.Sy - (yr0 - M));
     const float r2 = __builtin_amdgcn_rcpf(v.Sy - (yr0 - N)), r3 = __builtin_amdgcn_rcpf(v.Sy - (yr0 + N));
@@ -737,14 +740,15 @@ __device__ inline bool bp_column_view(const BpView& v, const float* __restrict__
     // chunk lies on one side of the source height (uniform across the workgroup) both ends are
     // affine in i, and g2 - g1 = -Bv (smx - smn) has the sign of -Bv for every voxel.
     const float e0 = v.Sz - Z0;
-    const float eL = e0 - (float)(ZC - 1) * dz;
+    const float eL = fmaf(-(float)(ZC - 1), dz, e0);
     const bool above = fminf(e0, eL) - hz >= 0.0f;
     const bool lin = above || (fmaxf(e0, eL) + hz <= 0.0f);
     float Alo, Dlo, Ahi, Dhi;
     {
         const float r1 = above ? rmin : rmax, r2 = above ? rmax : rmin;
-        const float A1 = v.Av - v.Bv * r1 * (e0 - hz), D1 = v.Bv * r1 * dz;
-        const float A2 = v.Av - v.Bv * r2 * (e0 + hz), D2 = v.Bv * r2 * dz;
+        const float B1 = v.Bv * r1, B2 = v.Bv * r2;
+        const float A1 = fmaf(-B1, e0 - hz, v.Av), D1 = B1 * dz;
+        const float A2 = fmaf(-B2, e0 + hz, v.Av), D2 = B2 * dz;
         const bool g1_high = v.Bv > 0.0f;
         Alo = g1_high ? A2 : A1;
         Dlo = g1_high ? D2 : D1;
@@ -760,7 +764,7 @@ __device__ inline bool bp_column_view(const BpView& v, const float* __restrict__
             const float ep = e + hz, em = e - hz;
             const float smx = ep * (ep >= 0.0f ? rmax : rmin);
             const float smn = em * (em >= 0.0f ? rmin : rmax);
-            const float g1 = v.Av - v.Bv * smn, g2 = v.Av - v.Bv * smx;
+            const float g1 = fmaf(-v.Bv, smn, v.Av), g2 = fmaf(-v.Bv, smx, v.Av);
             lo = fminf(g1, g2);
             hi = fmaxf(g1, g2);
         }
@@ -1481,9 +1485,11 @@ __device__ void fft_lds(float2* buf, const float2* __restrict__ tw, bool inv) {
 // ConeBeamGPU's filtering (python/ConeBeamGPU.py:296-302) weighs at cell centres (off_u =
 // du R/(2 SDD), off_v = -dv R/(2 SDD)) and filters fliplr(row): the spectrum is real and
 // even, so that equals the mirror image of the filtered row, written with mirror = 1.
+// `in` and `out` may be the same buffer (in-place filtering): a block reads its two rows before
+// its first barrier and writes them after its last one, and no other block touches them.
 template <int LOG2N>
 __global__ void __launch_bounds__(kFftThreads) fdk_filter_kernel(
-    const float* __restrict__ in, float* __restrict__ out, const double* __restrict__ spec,
+    const float* in, float* out, const double* __restrict__ spec,
     const float2* __restrict__ tw, int n_rows, int nu, int nv, float R, float su, float ou,
     float sv, float ov, float out_scale, int mirror) {
     constexpr int N = 1 << LOG2N;
@@ -1819,9 +1825,10 @@ int ctr_dd_backproject(const ctr_geometry* g, const double* angles, const int32_
     if (tiles >= 2147483647LL) return CTR_ERR_INVALID;
     const int curved = g->detector_shape == CTR_DETECTOR_CURVED ? 1 : 0;
     const float fdx = (float)g->dx, fdy = (float)(-g->dy), fdz = (float)(-g->dz);
-    // weight zeroing, or a volume of one z chunk (a slice through the source height, as in a 2D
-    // fan, is on neither side of it): the general kernel does every view
-    if (zero_tol > 0.0f || tiles_z == 1) {
+    // weight zeroing, a volume of one z chunk (a slice through the source height, as in a 2D
+    // fan, is on neither side of it), or a caller's request for one pass (zero_tol < 0): the
+    // general kernel does every view
+    if (zero_tol != 0.0f || tiles_z == 1) {
         hipLaunchKernelGGL(HIP_KERNEL_NAME(dd_backproject_tile_kernel<ZC, kBpAll>), dim3((unsigned)tiles),
                            dim3(256), 0, s, proj, vol, vt, (uint8_t*)nullptr, g->nx, g->ny, g->nz, g->nu,
                            g->nv, g->n_views, x0, y0, z0, fdx, fdy, fdz, curved, scale, zero_tol, accumulate,

@@ -17,6 +17,8 @@ All functions accept one volume [nz, ny, nx] (projections [nView, nv, nu]) or a 
 """
 from __future__ import annotations
 
+import weakref
+
 import torch
 
 from . import projector as P
@@ -36,15 +38,18 @@ def _bp_ref(proj: torch.Tensor, geom: ScanGeometry, mode: str = "ref") -> torch.
     return out
 
 
-_NORM_CACHE: dict = {}
+# per geometry object: {(backprojector, device): (A 1, B 1)}, freed with the geometry
+_NORM_CACHE: "weakref.WeakKeyDictionary[ScanGeometry, dict]" = weakref.WeakKeyDictionary()
 
 
 def normalisers(geom: ScanGeometry, backprojector: str, device, eps: float):
-    """(row sums A 1, column sums B 1) for the chosen backprojector B; cached per geometry."""
-    key = (id(geom), backprojector, str(device))
-    hit = _NORM_CACHE.get(key)
-    if hit is not None and hit[0] is geom:
-        return hit[1], hit[2]
+    """(row sums A 1, column sums B 1) for the chosen backprojector B; cached per geometry object
+    (held weakly: the cache goes when the geometry does)."""
+    per_geom = _NORM_CACHE.setdefault(geom, {})
+    key = (backprojector, str(device))
+    hit = per_geom.get(key)
+    if hit is not None:
+        return hit
     ones_v = torch.ones(geom.vol_shape, device=device)
     ones_p = torch.ones(geom.proj_shape, device=device)
     a1 = P.dd_forward(ones_v, geom)
@@ -54,7 +59,7 @@ def normalisers(geom: ScanGeometry, backprojector: str, device, eps: float):
         b1 = P.dd_backproject(ones_p, geom, mode="edge")
     else:
         b1 = _bp_ref(ones_p, geom)
-    _NORM_CACHE[key] = (geom, a1, b1)
+    per_geom[key] = (a1, b1)
     return a1, b1
 
 
@@ -107,31 +112,24 @@ def sart(proj0: torch.Tensor, geom: ScanGeometry, n_iter: int, relaxation: float
     return x.detach()
 
 
+def _backward_differences(x: torch.Tensor):
+    """Backward differences along the last three axes, at every voxel that has a predecessor
+    along all three (the voxel (i, j, k) of ``x`` with i, j, k >= 1 sits at (i-1, j-1, k-1))."""
+    core = x[..., 1:, 1:, 1:]
+    return (core - x[..., :-1, 1:, 1:], core - x[..., 1:, :-1, 1:], core - x[..., 1:, 1:, :-1])
+
+
 def _tv_terms(img: torch.Tensor, eps: float):
-    """Per-interior-voxel TV gradient terms of python/ASD_POCS.py:55-72 (before normalisation)."""
-    c = img[..., 1:-1, 1:-1, 1:-1]
-    # own term: backward differences at the voxel
-    w1 = c - img[..., 0:-2, 1:-1, 1:-1]
-    w2 = c - img[..., 1:-1, 0:-2, 1:-1]
-    w3 = c - img[..., 1:-1, 1:-1, 0:-2]
-    t1 = (w1 + w2 + w3) / torch.sqrt(eps + w1 ** 2 + w2 ** 2 + w3 ** 2)
-    # neighbour at +1 along axis 0 (its backward differences)
-    n0 = img[..., 2:, 1:-1, 1:-1]
-    w4 = n0 - c
-    w5 = n0 - img[..., 2:, 0:-2, 1:-1]
-    w6 = n0 - img[..., 2:, 1:-1, 0:-2]
-    t2 = -w4 / torch.sqrt(eps + w4 ** 2 + w5 ** 2 + w6 ** 2)
-    n1 = img[..., 1:-1, 2:, 1:-1]
-    w7 = n1 - img[..., 0:-2, 2:, 1:-1]
-    w8 = n1 - c
-    w9 = n1 - img[..., 1:-1, 2:, 0:-2]
-    t3 = -w8 / torch.sqrt(eps + w7 ** 2 + w8 ** 2 + w9 ** 2)
-    n2 = img[..., 1:-1, 1:-1, 2:]
-    w10 = n2 - img[..., 0:-2, 1:-1, 2:]
-    w11 = n2 - img[..., 1:-1, 0:-2, 2:]
-    w12 = n2 - c
-    t4 = -w12 / torch.sqrt(eps + w10 ** 2 + w11 ** 2 + w12 ** 2)
-    return t1 + t2 + t3 + t4
+    """Gradient of TV(x) = sum_v sqrt(eps + |D x(v)|^2) (D: backward differences) at the interior
+    voxels c, before normalisation: the voxel's own term sum_k D_k x(c) / |D x(c)|_eps minus,
+    for each axis k, D_k x(c + e_k) / |D x(c + e_k)|_eps from its successor along k.  This is the
+    vector of python/ASD_POCS.py:55-72."""
+    d = _backward_differences(img)
+    norm = torch.sqrt(eps + d[0] ** 2 + d[1] ** 2 + d[2] ** 2)
+    own = ((d[0] + d[1] + d[2]) / norm)[..., :-1, :-1, :-1]
+    ratio = [dk / norm for dk in d]
+    succ = ratio[0][..., 1:, :-1, :-1] + ratio[1][..., :-1, 1:, :-1] + ratio[2][..., :-1, :-1, 1:]
+    return own - succ
 
 
 def _normalise_pad(dtv: torch.Tensor, eps: float) -> torch.Tensor:
@@ -151,11 +149,8 @@ def tv_gradient_autograd(img: torch.Tensor, eps: float = 1e-7) -> torch.Tensor:
     TV(x) = sum over voxels v with all backward neighbours of sqrt(eps + |grad_b x(v)|^2),
     restricted to interior voxels and normalised as the reference does."""
     x = img.detach().requires_grad_(True)
-    g = x[..., 1:, 1:, 1:]
-    d0 = g - x[..., :-1, 1:, 1:]
-    d1 = g - x[..., 1:, :-1, 1:]
-    d2 = g - x[..., 1:, 1:, :-1]
-    tv = torch.sqrt(eps + d0 ** 2 + d1 ** 2 + d2 ** 2)
+    d = _backward_differences(x)
+    tv = torch.sqrt(eps + d[0] ** 2 + d[1] ** 2 + d[2] ** 2)
     # the reference's sum covers the interior voxels and their +1 neighbours in one axis
     (grad,) = torch.autograd.grad(tv.sum(), x)
     inner = grad[..., 1:-1, 1:-1, 1:-1]

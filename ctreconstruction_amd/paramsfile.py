@@ -50,6 +50,8 @@ def read_params_file(filename: str, fan: bool = False, keys: tuple | None = None
     params = {k: ("" if k in ("DataPath", "precision") else 0) for k in keys}
     with open(filename) as f:
         for lineno, line in enumerate(f, 1):
+            if not line.strip():
+                continue  # blank lines (e.g. a trailing one) carry no key
             key = line.split(":")[0].strip()
             if key not in params and key != "ReconVolume":
                 raise ParamsFileError(f"Unknown variables: {filename}:{lineno}: {line.rstrip()!r}")

@@ -21,10 +21,27 @@ from . import projector as P
 from .geometry import ScanGeometry, merged_params
 
 
+VARIANTS = ("pycuda", "numba")
+
+
 class Reconstruction:
-    def __init__(self, params: dict, device=None):
+    """``variant`` picks which of the reference's two GPU classes to follow where they differ:
+    ``"pycuda"`` (python/Reconstruction_pycuda.py, the default) builds the forward projector's
+    voxel planes on ``ReconCenter`` (:792-794) and backprojects without weight zeroing;
+    ``"numba"`` (python/Reconstruction_numba.py) builds them on ``PhantomCenter`` (:835-837) and
+    zeroes backprojection weights below 1e-5 (python/GPUFuncs_numba.py:28-29, 45-46).  Both
+    backproject onto ``ReconCenter``."""
+
+    def __init__(self, params: dict, device=None, variant: str = "pycuda"):
+        if variant not in VARIANTS:
+            raise ValueError(f"variant must be one of {VARIANTS}, got {variant!r}")
+        self.variant = variant
         self.geom = ScanGeometry(params)
         g = self.geom
+        if variant == "numba":
+            self._fp_geom = ScanGeometry(dict(g.params, ReconCenter=[float(v) for v in g.params["PhantomCenter"]]))
+        else:
+            self._fp_geom = g
         self.params = g.params
         self.device = torch.device(device) if device is not None else torch.device("cuda")
         # attribute names of python/Reconstruction_pycuda.py:72-99
@@ -97,7 +114,7 @@ class Reconstruction:
         """Distance-driven forward projection of ``image`` into ``proj`` (:750-760)."""
         self._method_check()
         vol = self._to_device(self.image, self.geom.vol_shape, "image")
-        proj = P.dd_forward(vol, self.geom)
+        proj = P.dd_forward(vol, self._fp_geom)
         self.proj = proj if isinstance(self.image, torch.Tensor) else _host(proj)
 
     def Filtering(self):
@@ -113,7 +130,7 @@ class Reconstruction:
         """FDK-weighted DD backprojection of ``proj`` (times dtheta) into ``image`` (:310-318)."""
         self._method_check()
         proj = self._to_device(self.proj, self.geom.proj_shape, "proj")
-        vol = P.dd_backproject(proj, self.geom, mode="ref")
+        vol = P.dd_backproject(proj, self.geom, mode="ref", zero_tol=1e-5 if self.variant == "numba" else 0.0)
         self.image = vol if isinstance(self.proj, torch.Tensor) else _host(vol)
 
     # ------------------------------------------------------------------ helpers of the reference

@@ -123,7 +123,9 @@ int ctr_fdk_filter(const ctr_geometry* g, const double* spectrum, const float* p
  * Voxel-driven gather over all views in one launch, no atomics.
  * vol = (accumulate ? vol : 0) + sum_views scale * BP_view(proj); the reference's FDK uses
  * scale = dtheta (Q = proj * dtheta, :411).  Overlap weights below `zero_tol` are zeroed
- * (0 = never, as the pycuda kernel; 1e-5 = the numba kernels).
+ * (0 = never, as the pycuda kernel; 1e-5 = the numba kernels).  A negative `zero_tol` zeroes
+ * nothing either, but runs every (voxel, view) pair in one pass instead of a common-case pass
+ * followed by a completing pass (the same sums in another order: a check of that split).
  * mode CTR_BP_ADJOINT computes scale * A^T proj, the exact transpose of ctr_dd_forward (its
  * weights and its 1e-6 transverse zeroing; `zero_tol` is ignored); it needs `branch` and has no
  * counterpart in the reference, which only has the FDK-weighted backprojector. */

@@ -59,3 +59,26 @@ def voxelise(ellipsoids, xc, yc, zc, sub=1):
                     inside = np.einsum("...i,ij,...j->...", r, M, r) <= 1.0
                     vol += rho * inside
     return vol / sub ** 3
+
+
+def asymmetric_blobs(half):
+    """Three separated ellipsoids placed off-centre and without mirror symmetry in a cube of
+    half-extent ``half`` (mm): (centre, semi-axes, Euler angles, density)."""
+    h = float(half)
+    return [((0.40 * h, -0.30 * h, 0.35 * h), (0.20 * h, 0.14 * h, 0.16 * h), (0.3, 0.0, 0.0), 1.0),
+            ((-0.45 * h, 0.35 * h, -0.20 * h), (0.14 * h, 0.22 * h, 0.15 * h), (0.0, 0.4, 0.0), 0.7),
+            ((-0.10 * h, -0.50 * h, -0.45 * h), (0.16 * h, 0.12 * h, 0.12 * h), (0.0, 0.0, 0.5), 0.5)]
+
+
+def centres_of_mass(rec, xs, ys, zs, ellipsoids, grow=1.6):
+    """Centre of mass (x, y, z) of ``rec`` [nz, ny, nx] inside a box ``grow`` times each
+    ellipsoid's bounding box, one row per ellipsoid."""
+    Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")
+    out = []
+    for (c, axes, ang, rho) in ellipsoids:
+        r = grow * float(np.max(axes))
+        box = (np.abs(X - c[0]) <= r) & (np.abs(Y - c[1]) <= r) & (np.abs(Z - c[2]) <= r)
+        w = np.where(box, rec, 0.0)
+        m = w.sum()
+        out.append([(w * X).sum() / m, (w * Y).sum() / m, (w * Z).sum() / m])
+    return np.array(out)

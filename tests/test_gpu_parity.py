@@ -285,3 +285,123 @@ def test_legacy_forward_backward_entry_points(dev, tmp_path):
     e_b = _rel(bw.recon, ref_b)
     print(f"legacy entry points: forward {e_f:.2e}, backward {e_b:.2e}")
     assert e_f <= TOL and e_b <= TOL
+
+
+@pytest.mark.parametrize("mode", ["edge", "ref"])
+def test_fdk_of_forward_projection_puts_blobs_in_place(dev, mode):
+    """fdk(dd_forward(x)) of three off-centre, mutually asymmetric ellipsoids at 64^3 puts each
+    one's centre of mass where it is: ties the forward projector's detector lattice on the GPU to
+    the backprojector's orientation, which the oracle pins against exact line integrals.  The
+    reference grid (Q1) moves the object by -dv/2 * SAD/SDD along z."""
+    from tests import analytic
+    p = make_params(NumberOfImage=[64, 64, 64], ImagePixelSpacing=[0.5, 0.5, 0.5],
+                    NumberOfDetectorPixels=[96, 88], DetectorPixelSize=[0.8, 0.8], NumberOfViews=90,
+                    FilterType="ram-lak")
+    g = O.Geometry(p)
+    ell = analytic.asymmetric_blobs(16.0)
+    xs, ys, zs = g.centres()
+    vol = analytic.voxelise(ell, xs, ys, zs, sub=2).astype(np.float32)
+    geom = ctr.ScanGeometry(p)
+    rec = ctr.fdk(ctr.dd_forward(torch.from_numpy(vol).to(dev), geom), geom, mode=mode).cpu().numpy()
+    err = analytic.centres_of_mass(rec.astype(np.float64), xs, ys, zs, ell) - np.array([e[0] for e in ell])
+    if mode == "ref":
+        err[:, 2] += 0.5 * g.dv * g.sad / g.sdd
+    print(f"blob centres [{mode}], error in voxels:\n{err / g.dx}")
+    assert np.abs(err[:, :2]).max() < 0.25 * g.dx
+    assert np.abs(err[:, 2]).max() < (0.25 * g.dz if mode == "edge" else 0.1)
+
+
+@pytest.mark.parametrize("name", ["flat", "wide_footprint", "helical"])
+def test_forward_against_both_reference_weight_zeroings(dev, name):
+    """The pycuda kernels zero overlap weights below 1e-6 in both directions
+    (python/GPUFuncs_pycuda.py:434-449), the numba kernels below 1e-5 (python/GPUFuncs_numba.py:16-47).
+    The z-prefix projector zeroes transverse weights below 1e-6 and never splits z overlaps per
+    cell, so its deviation from each variant (and from no zeroing) is measured here."""
+    p = make_params(**CASES[name])
+    g = O.Geometry(p)
+    vol = _vol(g, 21)
+    out = ctr.dd_forward(torch.from_numpy(vol).to(dev), ctr.ScanGeometry(p)).cpu().numpy()
+    errs = {tol: _rel(out, O.forward_project(g, vol.astype(np.float64), zero_tol=tol))
+            for tol in (1e-6, 1e-5, None)}
+    print(f"forward[{name}] vs oracle zeroing 1e-6 / 1e-5 / none: "
+          + " / ".join(f"{e:.2e}" for e in errs.values()))
+    assert max(errs.values()) <= TOL
+
+
+def test_numba_variant_planes_on_phantom_centre(dev):
+    """Reconstruction(params, variant="numba"): the forward projector's planes sit on PhantomCenter
+    (python/Reconstruction_numba.py:835-837), the backprojection stays on ReconCenter with the
+    numba kernels' 1e-5 weight zeroing (python/GPUFuncs_numba.py:28-29, 45-46)."""
+    kw = dict(CASES["flat"], ReconCenter=[0.5, -0.3, 0.2], FilterType="ram-lak")
+    p = make_params(PhantomCenter=[-1.0, 0.75, -0.5], **kw)
+    g_fp = O.Geometry(make_params(**dict(kw, ReconCenter=[-1.0, 0.75, -0.5])))
+    g_bp = O.Geometry(p)
+    vol = _vol(g_bp, 22)
+    R = ctr.Reconstruction(p, variant="numba")
+    R.image = vol
+    R.forward()
+    ref_p = O.forward_project(g_fp, vol.astype(np.float64))
+    e_f = _rel(R.proj, ref_p)
+    R.backward()  # unfiltered: the backprojector alone
+    ref_b = O.backproject(g_bp, np.asarray(R.proj, np.float64), "ref", zero_tol=1e-5)
+    e_b = _rel(R.image, ref_b)
+    # the default (pycuda) variant keeps ReconCenter for both
+    R2 = ctr.Reconstruction(p)
+    R2.image = vol
+    R2.forward()
+    e_p = _rel(R2.proj, O.forward_project(g_bp, vol.astype(np.float64)))
+    print(f"numba variant: forward {e_f:.2e}, backprojection {e_b:.2e}; pycuda variant forward {e_p:.2e}")
+    assert e_f <= TOL and e_b <= TOL and e_p <= TOL
+    assert _rel(R.proj, O.forward_project(g_bp, vol.astype(np.float64))) > 10 * TOL
+
+
+# Volumes of three 16-voxel z chunks whose (tile, view) pairs fall partly outside the
+# backprojector's common case, so that the completing pass has work:
+SPLIT_CASES = {
+    # a chunk spans 16 * 0.5 * mag / 0.55 = 21.6 .. 22.1 rows (+1 for its partial end rows):
+    # some chunks fit the 22-row limit of the row-integral scratch, some do not
+    "rows_over_22": dict(NumberOfImage=[32, 32, 48], NumberOfDetectorPixels=[40, 80],
+                         DetectorPixelSize=[1.0, 0.55], NumberOfViews=10, StartAngle=0.05,
+                         EndAngle=0.05 + 2 * np.pi),
+    # a voxel covers 0.5 * mag / 0.3 = 2.5 (axis-aligned) .. 3.5 (diagonal) cells: more than 3
+    # u taps at some views; a 16 x 16 tile's window is 40 .. 57 cells (+2 of margin) by about
+    # 9 rows (+2), so some windows fit the 576-cell LDS buffer and some do not
+    "u_taps_and_windows": dict(NumberOfImage=[32, 32, 48], NumberOfDetectorPixels=[120, 32],
+                               DetectorPixelSize=[0.3, 1.4], NumberOfViews=10, StartAngle=0.05,
+                               EndAngle=0.05 + 2 * np.pi),
+}
+
+
+def test_split_cases_reach_the_completing_pass():
+    mag_lo, mag_hi = 1500.0 / (1000.0 + 8 * np.sqrt(2)), 1500.0 / (1000.0 - 8 * np.sqrt(2))
+    p = SPLIT_CASES["rows_over_22"]
+    rows = 16 * 0.5 / p["DetectorPixelSize"][1]
+    assert rows * mag_lo < 22 < rows * mag_hi + 1
+    p = SPLIT_CASES["u_taps_and_windows"]
+    du = p["DetectorPixelSize"][0]
+    assert 0.5 * mag_lo / du < 3 < 0.5 * np.sqrt(2) * mag_hi / du
+    rows = 16 * 0.5 * mag_hi / p["DetectorPixelSize"][1] + 2
+    assert (8 * mag_lo / du + 2) * rows < 576 < (8 * np.sqrt(2) * mag_hi / du + 2) * rows
+
+
+@pytest.mark.parametrize("name", sorted(SPLIT_CASES) + ["flat"])
+@pytest.mark.parametrize("mode", ["ref", "edge"])
+def test_two_pass_backprojection_equals_one_pass(dev, name, mode):
+    """The common-case pass plus the completing pass add every (voxel, view) pair exactly once:
+    they agree with the single pass over all cases (zero_tol < 0) up to the order of the sums,
+    and both agree with the oracle."""
+    kw = dict(CASES.get(name, SPLIT_CASES.get(name)))
+    if name == "flat":
+        kw["NumberOfImage"] = [24, 24, 48]
+    p = make_params(**kw)
+    g = O.Geometry(p)
+    geom = ctr.ScanGeometry(p)
+    q = torch.from_numpy(np.random.default_rng(23).random((g.nview, g.nv, g.nu)).astype(np.float32)).to(dev)
+    two = ctr.dd_backproject(q, geom, mode=mode).cpu().numpy()
+    one = ctr.dd_backproject(q, geom, mode=mode, zero_tol=-1.0).cpu().numpy()
+    ref = O.backproject(g, q.cpu().numpy(), mode)
+    e_split = float(np.abs(two - one).max() / np.abs(one).max())
+    e_ref = _rel(two, ref)
+    print(f"backprojection split [{name},{mode}]: two-pass vs one-pass {e_split:.2e}, vs oracle {e_ref:.2e}")
+    assert e_split <= 1e-6
+    assert e_ref <= TOL

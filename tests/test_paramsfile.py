@@ -138,3 +138,20 @@ def test_conebeam_loaddata_errors_follow_the_reference(tmp_path):
         np.arange(48, dtype=np.int32).tofile(str(data / f"{n}.dat"))
     cb.LoadData()
     np.testing.assert_array_equal(cb.proj[:, :, 1], np.arange(48).reshape(6, 8))
+
+
+def test_blank_lines_are_skipped(tmp_path):
+    # a trailing empty line (and blank lines between keys) carry no key; the reference stops
+    # reading at such a line after keeping every key read before it (python/ConeBeam.py:56-82)
+    f = tmp_path / "blank.txt"
+    f.write_text(CONE + "\r\n\r\n")
+    assert PF.read_params_file(str(f)) == PF.read_params_file(str(_write(tmp_path, CONE)))
+    lines = CONE.splitlines(keepends=True)
+    f.write_text("".join(lines[:3]) + "   \n" + "".join(lines[3:]))
+    assert PF.read_params_file(str(f))["NumberOfViews"] == 680.0
+
+
+def _write(tmp_path, text):
+    f = tmp_path / "plain.txt"
+    f.write_text(text)
+    return f
