@@ -25,7 +25,7 @@ EXPORTS = (
     "ctr_version", "ctr_strerror", "ctr_select_branches", "ctr_zero_padded_length",
     "ctr_dd_forward_workspace_bytes", "ctr_dd_forward",
     "ctr_fdk_filter_workspace_bytes", "ctr_filter_spectrum", "ctr_fdk_filter",
-    "ctr_dd_backproject_workspace_bytes", "ctr_dd_backproject",
+    "ctr_dd_backproject_workspace_bytes", "ctr_dd_backproject", "ctr_dd_backproject_static",
     "ctr_pixel_filter_spectrum", "ctr_pixel_filter",
     "ctr_pixel_backproject_workspace_bytes", "ctr_pixel_backproject",
     "ctr_conv3x3_bf16_forward", "ctr_conv3x3_bf16_wgrad_workspace_bytes", "ctr_conv3x3_bf16_wgrad",
@@ -36,6 +36,7 @@ CTR_ERR_INVALID = 1
 CTR_ERR_UNSUPPORTED = 2
 CTR_ERR_WORKSPACE = 3
 CTR_ERR_HIP = 4
+CTR_ERR_SCRATCH = 5
 
 CTR_DETECTOR_FLAT = 0
 CTR_DETECTOR_CURVED = 1
@@ -112,6 +113,7 @@ def _declare(lib):
         "ctr_dd_backproject_workspace_bytes": (c_size, [gp, c_int]),
         "ctr_dd_backproject": (c_int, [gp, c_vp, c_vp, c_vp, c_vp, c_int, c_flt, c_flt, c_int,
                                        c_vp, c_size, c_vp]),
+        "ctr_dd_backproject_static": (c_int, [gp, c_vp, c_vp, c_vp, c_int, c_flt, c_flt, c_int, c_vp]),
         "ctr_pixel_filter_spectrum": (c_int, [c_int, c_dbl, c_int, c_dbl, c_vp, c_vp]),
         "ctr_pixel_filter": (c_int, [gp, c_vp, c_vp, c_vp, c_int, c_vp, c_size, c_vp]),
         "ctr_pixel_backproject_workspace_bytes": (c_size, [gp]),

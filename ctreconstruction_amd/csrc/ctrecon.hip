@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 
 #include "ctrecon.h"
@@ -1544,6 +1545,33 @@ hipError_t launch_filter(int l2, dim3 grid, hipStream_t st, const float* in, flo
     return hipErrorInvalidValue;
 }
 
+// Scratch memory of ctr_dd_backproject_static, for callers that cannot allocate: the FDK
+// backprojection's per-view table and its (tile, view) flags.  The HIP runtime gives each GPU
+// its own copy when it loads the code object there, and keeps it until the process ends.  In
+// host code the name `scratch` is only a handle: its device address on the current GPU comes
+// from hipGetSymbolAddress, looked up once per device and kept in scratch_ptr.
+constexpr size_t kStaticScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mutex;
+void* scratch_ptr[kMaxDevices];
+
+// Device address of `scratch` on the current device (looked up on the first call there).
+int static_scratch(void** out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return CTR_ERR_HIP;
+    if (dev < 0 || dev >= kMaxDevices) return CTR_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (!scratch_ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return CTR_ERR_HIP;
+        scratch_ptr[dev] = p;
+    }
+    *out = scratch_ptr[dev];
+    return CTR_OK;
+}
+
 }  // namespace
 
 // ==========================================================================================
@@ -1561,6 +1589,9 @@ const char* ctr_strerror(int code) {
         case CTR_ERR_UNSUPPORTED: return "unsupported configuration (z-slab branch, detector shape or FFT length)";
         case CTR_ERR_WORKSPACE: return "workspace too small";
         case CTR_ERR_HIP: return "HIP runtime error";
+        case CTR_ERR_SCRATCH:
+            return "problem too large for the static scratch of ctr_dd_backproject_static (32 MiB: "
+                   "64 bytes per view plus one byte per 16x16x16 voxel tile and view)";
         default: return "unknown error";
     }
 }
@@ -1849,6 +1880,22 @@ int ctr_dd_backproject(const ctr_geometry* g, const double* angles, const int32_
                        dim3(256), 0, s, proj, vol, vt, flags, g->nx, g->ny, g->nz, g->nu, g->nv, g->n_views,
                        x0, y0, z0, fdx, fdy, fdz, curved, scale, zero_tol, accumulate, tiles_x, tiles_y, tiles_z);
     return status_from_hip(hipGetLastError());
+}
+
+int ctr_dd_backproject_static(const ctr_geometry* g, const double* angles, const float* proj,
+                              float* vol, int mode, float scale, float zero_tol, int accumulate,
+                              void* stream) {
+    int st = check_geometry(g);
+    if (st) return st;
+    if (mode != CTR_BP_FDK_REF && mode != CTR_BP_FDK_EDGE) return CTR_ERR_UNSUPPORTED;
+    const size_t need = ctr_dd_backproject_workspace_bytes(g, mode);
+    if (need > kStaticScratchBytes) return CTR_ERR_SCRATCH;
+    void* ws = nullptr;
+    if ((st = static_scratch(&ws))) return st;
+    // the same launches as ctr_dd_backproject: the per-view table is written by a kernel and
+    // the flags are cleared by a fill kernel on `stream` before use
+    return ctr_dd_backproject(g, angles, nullptr, proj, vol, mode, scale, zero_tol, accumulate, ws,
+                              need, stream);
 }
 
 }  // extern "C"

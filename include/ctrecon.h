@@ -31,7 +31,8 @@ enum ctr_status {
     CTR_ERR_INVALID = 1,      /* bad geometry, null pointer, size out of range            */
     CTR_ERR_UNSUPPORTED = 2,  /* z-slab branch, unknown detector shape, FFT length > 8192 */
     CTR_ERR_WORKSPACE = 3,    /* workspace smaller than the *_workspace_bytes() query     */
-    CTR_ERR_HIP = 4           /* a HIP launch or runtime call failed                       */
+    CTR_ERR_HIP = 4,          /* a HIP launch or runtime call failed                       */
+    CTR_ERR_SCRATCH = 5       /* too large for ctr_dd_backproject_static's static scratch  */
 };
 
 enum ctr_detector_shape { CTR_DETECTOR_FLAT = 0, CTR_DETECTOR_CURVED = 1 };
@@ -133,6 +134,21 @@ size_t ctr_dd_backproject_workspace_bytes(const ctr_geometry* g, int mode);
 int ctr_dd_backproject(const ctr_geometry* g, const double* angles, const int32_t* branch,
                        const float* proj, float* vol, int mode, float scale, float zero_tol,
                        int accumulate, void* workspace, size_t workspace_bytes, void* stream);
+
+/* No counterpart in the reference: ctr_dd_backproject (modes CTR_BP_FDK_REF and
+ * CTR_BP_FDK_EDGE) without a workspace argument, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Its scratch (the per-view table and the per-(tile, view) flags) lives in a static 32 MiB
+ * device array, one per GPU, held until the process ends; problems needing more
+ * (ctr_dd_backproject_workspace_bytes(g, mode) > 32 MiB, e.g. above about 1000 views at 512^3)
+ * return CTR_ERR_SCRATCH, and CTR_BP_ADJOINT returns CTR_ERR_UNSUPPORTED.  It allocates
+ * nothing and launches the same kernels as ctr_dd_backproject.  Every call on one GPU shares
+ * that GPU's array: calls that overlap in time on one GPU (other streams, other threads) race
+ * and must be serialised by the caller.  The current device must be the one `stream` and the
+ * buffers belong to. */
+int ctr_dd_backproject_static(const ctr_geometry* g, const double* angles, const float* proj,
+                              float* vol, int mode, float scale, float zero_tol, int accumulate,
+                              void* stream);
 
 /* ---- Pixel-driven cone-beam FDK ----------------------------------------------------------
  * Replaces ConeBeam.Reconstruction of python/ConeBeamGPU.py:203-321, the reference's
