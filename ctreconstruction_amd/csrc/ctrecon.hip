@@ -3,7 +3,7 @@
 // Kernels (all FP32 arithmetic; FP64 only in the per-view geometry kernels):
 //   fp_geometry_kernel          per-view DD footprint tables, FP64 -> FP32 (index space)
 //   zprefix_kernel              vol[z][y][x] -> z-prefix tables [y][x][k] and [x][y][k]
-//   dd_forward_column_kernel    one wavefront per (view, detector column, 63 rows)
+//   dd_forward_kernel           one wavefront per (view, 4 detector columns, 63 rows)
 //   bp_geometry_kernel          per-view backprojection constants, FP64 -> FP32
 //   dd_backproject_tile_kernel  one workgroup per 16 x 16 x ZC voxel tile, windows in LDS
 //   adj_geometry_kernel         per-view constants of the inverse central projection
@@ -244,34 +244,42 @@ __global__ void __launch_bounds__(256) zprefix_kernel(const float* __restrict__ 
 // are in index space (plane k at k), so the same code serves increasing (x) and decreasing
 // (y, z) planes.
 //
-// One wavefront per (view, detector column, kColRows rows).  All rays of a column share their
-// transverse footprint at every slab (the U-edge lines depend on the column only), so the
-// transverse overlaps, the slab test and the transverse addresses are uniform across the
-// wavefront: lane L works them out for slab n0 + L of a batch of 64 slabs, and the wavefront
-// then walks the batch reading them back from lane q as scalars.
+// One wavefront per (view, kFpCols adjacent detector columns, kColRows rows).  All rays of a
+// column share their transverse footprint at every slab (the U-edge lines depend on the column
+// only), so the transverse overlaps and the lines they cover are uniform across the wavefront.
+// In a batch of kFpBatch slabs, lane 4 s + c works them out for slab n0 + s and column c, the
+// four lanes of slab s agree (quad swizzles) on the union of their columns' lines and z
+// breakpoints, and the wavefront then walks the batch slab by slab.
 //
 // z direction: the footprint of row k at a slab is [c_k, c_k+1] with c_e = nu_e tt + gamma on
 // the row edges e, and its overlap-weighted sum over a z line is the line's z integral
-// I(c_k+1) - I(c_k), interpolated in the z-prefix table (one 8-byte lookup per edge and line,
-// the two breakpoints around the edge).  Lane
-// L owns edge e0 + L of kColRows + 1 = 64 consecutive edges and sums, in FP64 over the slabs,
-// H_e = sum_slab sum_a w_a I_a(c_e) with the transverse weights w_a pre-divided by wa * tt; row
-// k is then (H_k+1 - H_k) / (nu_k+1 - nu_k), scaled by the ray's length factor.  The width
-// |c_k+1 - c_k| = |nu_k+1 - nu_k| |tt| is the reference's normaliser; a footprint inside one
-// cell gets weight 1 (up to rounding) as in the reference, and cells outside the volume are
-// cut by clamping c_e to [0, nz].  Transverse overlaps below zero_tol times the footprint width
-// are dropped as in the reference; z overlaps are not split per cell, so the reference's
+// I(c_k+1) - I(c_k), interpolated in the z-prefix table (the two breakpoints around the edge).
+// Per slab the wavefront stages the z segment of the union of its columns' lines in LDS with
+// one to three coalesced 16-byte-per-lane loads (lanes past the segment get an out-of-range
+// offset and fetch nothing): adjacent columns share most of their lines, so one load serves
+// about two columns.  Lane L owns edge e0 + L of kColRows + 1 = 64 consecutive edges and sums,
+// in FP64 over the slabs and per column, H_e = sum_slab sum_a w_a I_a(c_e) with the transverse
+// weights w_a pre-divided by wa * tt; row k is then (H_k+1 - H_k) / (nu_k+1 - nu_k), scaled by
+// the ray's length factor, and the four columns of a row leave in one 16-byte store.  The
+// width |c_k+1 - c_k| = |nu_k+1 - nu_k| |tt| is the reference's normaliser; a footprint inside
+// one cell gets weight 1 (up to rounding) as in the reference, and cells outside the volume
+// are cut by clamping c_e to [0, nz].  Transverse overlaps below zero_tol times the footprint
+// width are dropped as in the reference; z overlaps are not split per cell, so the reference's
 // per-cell zeroing of z weights below zero_tol (1e-6 of the footprint) has no counterpart.
-constexpr int kColWaves = 4;
-constexpr int kColRows = 63;  // detector rows per wavefront (64 lanes = its 64 row edges)
-#ifndef CTR_FP_UNROLL
-#define CTR_FP_UNROLL 2
+// Footprints over three cells, and unions too large for the LDS segment, take an explicit
+// per-line path that reads the breakpoint pairs straight from memory.
+constexpr int kFpCols = 4;              // detector columns per wavefront
+constexpr int kFpBatch = 64 / kFpCols;  // slabs per batch (one lane per slab and column)
+constexpr int kColRows = 63;            // detector rows per wavefront (64 lanes = its 64 row edges)
+#ifndef CTR_FP_WAVES
+#define CTR_FP_WAVES 4
 #endif
-constexpr int kFpUnroll = CTR_FP_UNROLL;  // slabs whose segment loads are in flight together
-// z segment of a line staged per slab: 21 lanes x 4 breakpoints per load; the wavefront's
-// three lines arrive in one or two coalesced 16-byte-per-lane loads
-constexpr int kSegLanes = 21;
-constexpr int kSeg = 4 * kSegLanes;
+constexpr int kFpWaves = CTR_FP_WAVES;  // wavefronts per workgroup: 16 adjacent columns, 64 bytes of a row
+#ifndef CTR_FP_MIN_WAVES
+#define CTR_FP_MIN_WAVES 5  // resident wavefronts per SIMD the register budget must allow
+#endif
+constexpr int kFpMaxLoads = 3;          // segment loads per slab
+constexpr int kFpMaxQuads = 64 * kFpMaxLoads;
 // Views per block of the workgroup order: consecutive workgroups on one XCD take kViewBlock
 // adjacent views of one column group, then the XCD's next column group, so the workgroups
 // resident on an XCD at one time trace nearby rays and share volume lines in its L2.
@@ -295,7 +303,22 @@ __device__ inline float lane_f(float v, int q) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), q));
 }
 
-__global__ void __launch_bounds__(256) dd_forward_column_kernel(
+// value of lane ^ 1 and lane ^ 2 inside each group of four lanes (DPP quad permutes)
+__device__ inline int quad_xor1(int v) { return __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false); }
+__device__ inline int quad_xor2(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false); }
+__device__ inline int quad_min(int v) {
+    v = min(v, quad_xor1(v));
+    return min(v, quad_xor2(v));
+}
+__device__ inline int quad_max(int v) {
+    v = max(v, quad_xor1(v));
+    return max(v, quad_xor2(v));
+}
+
+// per (slab, column) record of a batch: packed flags (see below) and the weights
+constexpr int kRecHit = 1 << 10;
+
+__global__ void __launch_bounds__(64 * kFpWaves, CTR_FP_MIN_WAVES) dd_forward_kernel(
     const float* __restrict__ t_ys, const float* __restrict__ t_xs, float* __restrict__ proj,
     const float* __restrict__ tabs, const float* __restrict__ rows,
     const int32_t* __restrict__ branch, int nx, int ny, int nz, int nzp, int nu, int nv, int stride,
@@ -303,74 +326,74 @@ __global__ void __launch_bounds__(256) dd_forward_column_kernel(
     // workgroup b runs on XCD b % 8, which takes the runs of kXcdRun column groups
     // cg = (8 c + xcd) kXcdRun + m: every XCD gets an equal share of every band of detector
     // rows (the rows outside the volume's shadow are cheap).  On one XCD the local index runs
-    // over (row block, view block, c, m, view in block), fastest last.
+    // over (row block, view block, c, view in block, m), fastest last.
     const int b = blockIdx.x;
     const int xcd = b & 7, l = b >> 3;
     const int runs = (col_groups + 8 * kXcdRun - 1) / (8 * kXcdRun);
-    const int vi = l % kViewBlock;
-    const int r1 = l / kViewBlock;
-    const int cg = (((r1 / kXcdRun) % runs) * 8 + xcd) * kXcdRun + r1 % kXcdRun;
-    const int r2 = r1 / (kXcdRun * runs);
+    // the kXcdRun column groups of a run are consecutive workgroups: they write adjacent parts
+    // of the same detector rows at the same time
+    const int m = l % kXcdRun;
+    const int l2 = l / kXcdRun;
+    const int vi = l2 % kViewBlock;
+    const int r1 = l2 / kViewBlock;
+    const int cg = ((r1 % runs) * 8 + xcd) * kXcdRun + m;
+    const int r2 = r1 / runs;
     const int view = (r2 % n_vblocks) * kViewBlock + vi;
     const int rb = r2 / n_vblocks;
     if (view >= n_views || cg >= col_groups) return;
     const int lane = threadIdx.x & 63;
     // wave-uniform: lets the compiler keep column data and addresses in scalar registers
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
-    const int j = cg * kColWaves + wave;
-    if (j >= nu) return;
+    const int j0 = (cg * kFpWaves + wave) * kFpCols;  // first column of this wavefront
+    if (j0 >= nu) return;
     const int e0 = rb * kColRows;
     const int k = e0 + lane;  // this lane's row (lanes 0 .. kColRows-1) and edge
     const bool active = lane < kColRows && k < nv;
     const int kk = min(k, nv - 1);
-    const size_t out_idx = ((size_t)view * nv + kk) * nu + j;
     const int br = branch[view];
+    float* __restrict__ out_row = proj + ((size_t)view * nv + kk) * nu + j0;
+    const bool vec_store = (nu & 3) == 0 && j0 + kFpCols <= nu && (reinterpret_cast<uintptr_t>(proj) & 15) == 0;
     if (br != CTR_BRANCH_X && br != CTR_BRANCH_Y) {
-        if (active) proj[out_idx] = 0.0f;
+        if (active) {
+#pragma unroll
+            for (int c = 0; c < kFpCols; ++c)
+                if (j0 + c < nu) out_row[c] = 0.0f;
+        }
         return;
     }
     const bool xs = (br == CTR_BRANCH_X);
     const float* __restrict__ src = xs ? t_xs : t_ys;
     const int ns = xs ? nx : ny;
     const int nt = xs ? ny : nx;
-    // byte offsets inside a batch of 64 slabs fit in 31 bits (host check)
+    // byte offsets inside a batch of slabs fit in 31 bits (host check)
     const int line_bytes = 4 * nzp;
     const int slab_bytes = line_bytes * nt;
     const char* __restrict__ src_bytes = reinterpret_cast<const char*>(src);
     const float* T = tabs + (size_t)view * stride;
-    const float a_al = T[j], b_al = T[j + 1];
-    const float a_be = T[nu + 1 + j], b_be = T[nu + 2 + j];
-    const float T0 = T[2 * nu + 2 + j], dT = T[3 * nu + 2 + j];
-    const float sec = T[4 * nu + 2 + j], inv = T[5 * nu + 2 + j];
     const float gam = T[6 * nu + 2];
     const float nue = rows[min(k, nv)];  // this lane's edge slope
-    const float vc = rows[nv + 1 + kk];
-    const float s2 = vc * inv;
-    const float Lk = sec * sqrtf(1.0f + s2 * s2) / (rows[kk + 1] - rows[kk]);
     const float fnz = (float)nz, fnt = (float)nt;
     // z extent of the wavefront's edges
     const float nu_first = rows[e0], nu_last = rows[min(e0 + kColRows, nv)];
     // gamma in a vector register: the edge fma then reads one scalar operand (tt)
     float gam_v;
     asm volatile("v_mov_b32 %0, %1" : "=v"(gam_v) : "s"(gam));
-    double H = 0.0;
-    // the wavefront's LDS segment buffer.  With NL loads per slab (QL = NL kSegLanes quads per
-    // line), quad g = 64 i + L (load i, lane L) holds breakpoints 4 j .. 4 j + 3 of line l at
-    // bytes 16 (QL l + j); quads past the three lines repeat the last one
-    __shared__ uint4 segs[kColWaves][128];
+    // as (slab, column) worker: column j0 + cl of slab n0 + (lane >> 2)
+    const int cl = lane & (kFpCols - 1);
+    const int jc = min(j0 + cl, nu - 1);
+    const bool col_ok = j0 + cl < nu;
+    const float a_al = T[jc], b_al = T[jc + 1];
+    const float a_be = T[nu + 1 + jc], b_be = T[nu + 2 + jc];
+    const float T0 = T[2 * nu + 2 + jc], dT = T[3 * nu + 2 + jc];
+    double H[kFpCols];
+#pragma unroll
+    for (int c = 0; c < kFpCols; ++c) H[c] = 0.0;
+    __shared__ uint4 segs[kFpWaves][kFpMaxQuads];
+    __shared__ float4 recs[kFpWaves][64];
     uint4* __restrict__ seg = segs[wave];
-    const char* seg_bytes = reinterpret_cast<const char*>(seg);
-    int seg_l[2][2], seg_j16[2][2];
-#pragma unroll
-    for (int nl = 1; nl <= 2; ++nl) {
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            const int ql = nl * kSegLanes, g = 64 * i + lane;
-            const int l = min(g / ql, 2);
-            seg_l[nl - 1][i] = l;
-            seg_j16[nl - 1][i] = 16 * min(g - ql * l, ql - 1);
-        }
-    }
+    const char* segs_bytes = reinterpret_cast<const char*>(&segs[0][0]);
+    const int seg_off = wave * kFpMaxQuads * 16;  // this wavefront's segment buffer in segs
+    const float4* rec = recs[wave];
     // this lane's edge at the slab with fraction tt: its table entry's byte offset in a line
     // and the fractional part of the clamped edge
     auto edge = [&](float tt, unsigned& off, float& fr) {
@@ -379,166 +402,252 @@ __global__ void __launch_bounds__(256) dd_forward_column_kernel(
         fr = c - fl;
         off = 4u * (unsigned)fl;
     };
-    for (int n0 = 0; n0 < ns; n0 += 64) {
+    // lanes g = 64 i + lane of segment load i: (g + 1/2) as a float, for the line index
+    float gh[kFpMaxLoads];
+#pragma unroll
+    for (int i = 0; i < kFpMaxLoads; ++i) gh[i] = (float)(64 * i + lane) + 0.5f;
+    // Can any slab hit?  The footprint lines and tt are affine in the slab index, so their
+    // values at the first and last slab bound every slab's; a wavefront whose four columns
+    // miss the volume (transversely, or in z for all of its rows) writes zeros.
+    bool may_hit = false;
+    {
+        const float fN = (float)(ns - 1);
+        const float a1N = fmaf(a_be, fN, a_al), a2N = fmaf(b_be, fN, b_al);
+        const float lo = fminf(fminf(a_al, b_al), fminf(a1N, a2N));
+        const float hi = fmaxf(fmaxf(a_al, b_al), fmaxf(a1N, a2N));
+        const float ttN = fmaf(dT, fN, T0);
+        const float z1 = fmaf(nu_first, T0, gam), z2 = fmaf(nu_last, T0, gam);
+        const float z3 = fmaf(nu_first, ttN, gam), z4 = fmaf(nu_last, ttN, gam);
+        const float zmin = fminf(fminf(z1, z2), fminf(z3, z4)), zmax = fmaxf(fmaxf(z1, z2), fmaxf(z3, z4));
+        may_hit = col_ok && hi >= -1.0f && lo <= fnt + 1.0f && zmax >= -1.0f && zmin <= fnz + 1.0f;
+    }
+    const int n_end = __ballot(may_hit) ? ns : 0;
+    for (int n0 = 0; n0 < n_end; n0 += kFpBatch) {
         // the batch's slabs through one buffer resource: a line is addressed by a scalar byte
-        // offset, the lane's breakpoint pair by a vector one
+        // offset, the lane's quad by a vector one
         const char* batch = src_bytes + (size_t)n0 * slab_bytes;
         const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-            (void*)batch, 0, (int)((size_t)min(64, ns - n0) * slab_bytes), 0x00020000);
-        auto pair = [&](unsigned off, int line_off) {
-            const v2u v = __builtin_amdgcn_raw_buffer_load_b64(rsrc, (int)off, line_off, 0);
-            PTaps t;
-            t.a = __uint_as_float(v.x);
-            t.b = __uint_as_float(v.y);
-            return t;
-        };
-        // lane L: transverse footprint of slab n0 + L.  Mode 0: misses the volume (transverse,
-        // or in z for every edge of the wavefront); 1 and 3: at most three cells (weights
-        // pre-divided by wa tt, first line at byte offset b_o0 in the batch) and the edges'
-        // breakpoints inside a segment staged with one (1) or two (3) loads (its start and da
-        // in b_kd); 2: otherwise, explicit loop.
-        float b_tt = 1.0f, b_w0 = 0.0f, b_w1 = 0.0f, b_w2 = 0.0f, b_alo = 0.0f, b_ahi = 0.0f;
-        int b_o0 = 0, b_kd = 0, b_mode = 0;
+            (void*)batch, 0, (int)((size_t)min(kFpBatch, ns - n0) * slab_bytes), 0x00020000);
+        // lane 4 s + c: slab n0 + s, column j0 + c.  Packed record p: first line relative to
+        // the slab's union (bits 0-7), hit (bit 10).  Union u1: first line (bits 0-23), lines
+        // staged (24-31); u2: first z breakpoint (0-15), quads per line (16-23), walk (24-27:
+        // 0 no column hits, 1-3 staged with that many segment loads and two lines per column,
+        // 4-6 the same with three lines per column, 7 explicit).
+        int p = 0, u1 = 0, u2 = 0;
         {
-            const int n = n0 + lane;
-            if (n < ns) {
+            const int n = n0 + (lane >> 2);
+            float tt = 1.0f, w0 = 0.0f, w1 = 0.0f, w2 = 0.0f;
+            int kas = 0x7fffffff, kend2 = -1, kend3 = -1, zlo = 0x7fffffff, zhi = -1, da = -1;
+            if (col_ok && n < ns) {
                 const float fn = (float)n;
                 const float a1 = fmaf(a_be, fn, a_al), a2 = fmaf(b_be, fn, b_al);
                 const float alo = fminf(a1, a2), ahi = fmaxf(a1, a2);
-                const int kas = max((int)floorf(alo), 0), kae = min((int)floorf(ahi), nt - 1);
-                const float tt = fmaf(dT, fn, T0);
+                const int ks = max((int)floorf(alo), 0), ke = min((int)floorf(ahi), nt - 1);
+                tt = fmaf(dT, fn, T0);
                 const float za = fmaf(nu_first, tt, gam), zb = fmaf(nu_last, tt, gam);
                 const bool z_hit = fmaxf(za, zb) >= -0.01f && fminf(za, zb) <= fnz + 0.01f;
-                b_tt = tt;
-                b_alo = alo;
-                b_ahi = ahi;
-                if (kas <= kae && z_hit) {
-                    const int da = kae - kas;
+                if (ks <= ke && z_hit) {
+                    da = ke - ks;
                     const float wa = ahi - alo;
-                    // overlaps with cells kas .. kas+2 (floor(max(alo, 0)) == kas); negative
+                    // overlaps with cells ks .. ks+2 (floor(max(alo, 0)) == ks); negative
                     // values past the footprint fail the threshold test
                     const float lo = fmaxf(alo, 0.0f), hi = fminf(ahi, fnt);
-                    const float s1 = (float)kas + 1.0f;
+                    const float s1 = (float)ks + 1.0f;
                     float o0 = fminf(hi, s1) - lo, o1 = fminf(hi, s1 + 1.0f) - s1, o2 = hi - (s1 + 1.0f);
                     const float tl = zero_tol * wa;
                     o0 = (o0 >= tl) ? o0 : 0.0f;
                     o1 = (o1 >= tl) ? o1 : 0.0f;
                     o2 = (o2 >= tl) ? o2 : 0.0f;
                     const float r = __builtin_amdgcn_rcpf(wa * tt);
-                    b_w0 = o0 * r;
-                    b_w1 = o1 * r;
-                    b_w2 = o2 * r;
-                    b_o0 = lane * slab_bytes + kas * line_bytes;
+                    w0 = o0 * r;
+                    w1 = o1 * r;
+                    w2 = o2 * r;
                     // breakpoints of the wavefront's edges: floor of the clamped end edges, the
                     // same arithmetic as each lane's (the edges are monotone in the lane)
                     const int kf = (int)floorf(__builtin_amdgcn_fmed3f(fmaf(nu_first, tt, gam), 0.0f, fnz));
                     const int kl = (int)floorf(__builtin_amdgcn_fmed3f(fmaf(nu_last, tt, gam), 0.0f, fnz));
-                    // the segment starts at a multiple of 4 breakpoints (16-byte aligned loads);
-                    // past the end of a line it reads the next one (or the table's slack),
-                    // which no lane uses
-                    const int kseg = min(kf, kl) & ~3, span = max(kf, kl) - kseg;
-                    const int nl = (span <= kSeg - 2) ? 1 : (span <= 2 * kSeg - 2) ? 2 : 0;
-                    // quads per line that hold breakpoints kseg .. kmax + 1 (<= 21 nl)
-                    const int nq = (span + 5) >> 2;
-                    b_kd = kseg | (min(da, 2) << 16) | (nq << 18);  // kseg <= nz < 2^16 (host check)
-                    b_mode = (da > 2 || nl == 0) ? 2 : (nl == 1) ? 1 : 3;
+                    kas = ks;
+                    kend2 = ks + 1;
+                    kend3 = ks + 2;
+                    zlo = min(kf, kl);
+                    zhi = max(kf, kl);
                 }
             }
+            recs[wave][lane] = make_float4(tt, w0, w1, w2);
+            const int maxda = quad_max(da);
+            const int L0 = quad_min(kas);
+            // every column reads two (three) consecutive lines from its first one; lines past
+            // its footprint have weight 0 but are staged, so that every read is of staged data
+            const int L1 = (maxda >= 2) ? quad_max(kend3) : quad_max(kend2);
+            const int zl = quad_min(zlo), zh = quad_max(zhi);
+            if (maxda >= 0) {
+                // the segment starts at a multiple of 4 breakpoints (16-byte aligned loads) and
+                // holds breakpoints kseg .. zh + 1; past the end of a line it reads the next one
+                // (or the table's slack, or nothing past the batch), which no weight uses
+                const int kseg = zl & ~3;
+                const int nq = (zh - kseg + 5) >> 2;
+                const int nlines = L1 - L0 + 1;
+                const int nl = (nlines * nq + 63) >> 6;
+                const bool staged = maxda <= 2 && nl <= kFpMaxLoads;
+                const int walk = !staged ? 7 : (maxda >= 2 ? 3 + nl : nl);
+                u1 = staged ? (L0 | (nlines << 24)) : 0;
+                u2 = kseg | (min(nq, 255) << 16) | (walk << 24);  // kseg <= nz < 2^16
+                // staged: kas - L0 < nlines <= kFpMaxQuads fits its 8 bits
+                if (da >= 0) p = (staged ? kas - L0 : 0) | kRecHit;
+            }
         }
-        // mode-1 slabs, software-pipelined: the lookups of the next slab are loaded while the
-        // current one is summed
-        // mode-1 and mode-3 slabs
-        auto walk = [&](auto nl, uint64_t mask) {
-            constexpr int NL = decltype(nl)::value;  // loads per slab
-            constexpr int QL = NL * kSegLanes;      // quads per line
+        const uint64_t heads = __ballot(cl == 0);
+        // staged slabs, software-pipelined in pairs: both slabs' segment loads go out, then both
+        // are summed; no load result lives across rounds, so nothing waits to be copied
+        auto walk = [&](auto nl_c, auto lines_c, uint64_t mask) {
+            constexpr int NL = decltype(nl_c)::value;        // segment loads per slab
+            constexpr int LINES = decltype(lines_c)::value;  // lines read per column
             struct Set {
                 v4u d[NL];
-                float w[3], fr;
-                unsigned rel;  // byte offset of the lane's breakpoint pair in a line's segment
+                int q;
             };
             auto issue = [&](int q, Set& S) {
-                const float tt = lane_f(b_tt, q);
-                unsigned off;
-                edge(tt, off, S.fr);
-                const int kd = __builtin_amdgcn_readlane(b_kd, q);
-                const int kseg = kd & 0xffff, da = (kd >> 16) & 3, nq16 = (kd >> 18) << 4;
-                const int base = __builtin_amdgcn_readlane(b_o0, q) + 4 * kseg;
-                // quads past the breakpoints in use get an offset beyond the buffer: the range
-                // check returns 0 for them without a memory access.  The line index is at most 2
-                // and a line below 2^23 bytes (nz <= 65535): a full-rate 24-bit multiply
+                S.q = q;
+                const int U1 = __builtin_amdgcn_readlane(u1, 4 * q), U2 = __builtin_amdgcn_readlane(u2, 4 * q);
+                const int L0 = U1 & 0xffffff, nlines = (unsigned)U1 >> 24;
+                const int kseg = U2 & 0xffff, nq = (U2 >> 16) & 0xff;
+                // quad g of the segment: line g / nq (the quotient from an FP32 reciprocal:
+                // (g + 1/2)/nq is at least 1/(2 nq) from an integer and its rounding error below
+                // 1e-4, so the floor is exact for g < 192, nq < 256), quad g - line nq, at byte
+                // line * line_bytes + 16 quad = 16 g + line (line_bytes - 16 nq)
+                const float inv_nq = __builtin_amdgcn_rcpf((float)nq);
+                const int step = line_bytes - 16 * nq;
+                const int soff = q * slab_bytes + L0 * line_bytes + 4 * kseg;
 #pragma unroll
-                for (int i = 0; i < NL; ++i)
-                    S.d[i] = __builtin_amdgcn_raw_buffer_load_b128(
-                        rsrc, seg_j16[NL - 1][i] < nq16 ? __mul24(min(seg_l[NL - 1][i], da), line_bytes) + seg_j16[NL - 1][i]
-                                                        : 0x7ffffff0, base, 0);
-                S.rel = off - 4u * (unsigned)kseg;
-                S.w[0] = lane_f(b_w0, q);
-                S.w[1] = lane_f(b_w1, q);
-                S.w[2] = lane_f(b_w2, q);
+                for (int i = 0; i < NL; ++i) {
+                    const int ln = (int)(gh[i] * inv_nq);
+                    // quads past the segment get an offset beyond the buffer: the range check
+                    // returns 0 for them without a memory access
+                    const int voff = ln < nlines ? 16 * (64 * i + lane) + __mul24(ln, step) : 0x7ffffff0;
+                    S.d[i] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, voff, soff, 0);
+                }
             };
             auto consume = [&](const Set& S) {
 #pragma unroll
                 for (int i = 0; i < NL; ++i) seg[64 * i + lane] = make_uint4(S.d[i].x, S.d[i].y, S.d[i].z, S.d[i].w);
-                const PTaps p0 = *reinterpret_cast<const PTaps*>(seg_bytes + S.rel);
-                const PTaps p1 = *reinterpret_cast<const PTaps*>(seg_bytes + 16 * QL + S.rel);
-                const PTaps p2 = *reinterpret_cast<const PTaps*>(seg_bytes + 32 * QL + S.rel);
-                const float i0 = fmaf(S.fr, p0.b - p0.a, p0.a);
-                const float i1 = fmaf(S.fr, p1.b - p1.a, p1.a);
-                const float i2 = fmaf(S.fr, p2.b - p2.a, p2.a);
-                H += (double)fmaf(S.w[0], i0, fmaf(S.w[1], i1, S.w[2] * i2));
-            };
-            // kFpUnroll slabs per round: their loads go out together, then they are summed in
-            // order; no load result lives across rounds, so nothing waits to be copied.  Every
-            // round loads and sums kFpUnroll slabs (past the last one it repeats that slab with
-            // weight 0), so the loads in flight, and with them each wait, are the same on all
-            // paths
-            while (mask) {
-                Set S[kFpUnroll];
-                int q = 0;
+                const int U2 = __builtin_amdgcn_readlane(u2, 4 * S.q);
+                const int kseg = U2 & 0xffff, nq16 = (U2 >> 12) & 0xff0;
+                float4 r[kFpCols];
 #pragma unroll
-                for (int u = 0; u < kFpUnroll; ++u) {
-                    const bool more = mask != 0;
-                    if (more) {
-                        q = __builtin_ctzll(mask);
-                        mask &= mask - 1;
-                    }
-                    issue(q, S[u]);
-                    if (!more) S[u].w[0] = S[u].w[1] = S[u].w[2] = 0.0f;
+                for (int c = 0; c < kFpCols; ++c) r[c] = rec[4 * S.q + c];
+                unsigned lb[kFpCols];
+                float fr[kFpCols];
+#pragma unroll
+                for (int c = 0; c < kFpCols; ++c) {
+                    const int pc = __builtin_amdgcn_readlane(p, 4 * S.q + c);
+                    // byte of the column's first line in the segment buffers, less the segment's
+                    // first breakpoint (scalar): one shift-add per lane
+                    const unsigned sbase = (unsigned)(seg_off + (pc & 0xff) * nq16 - 4 * kseg);
+                    const float cz = __builtin_amdgcn_fmed3f(fmaf(nue, r[c].x, gam_v), 0.0f, fnz);
+                    const float fl = floorf(cz);
+                    fr[c] = cz - fl;
+                    // a column that misses this slab reads the segment's start with weight 0
+                    lb[c] = (pc & kRecHit) ? 4u * (unsigned)fl + sbase : (unsigned)seg_off;
                 }
-                __builtin_amdgcn_sched_barrier(0);
+                PTaps t[kFpCols][LINES];
 #pragma unroll
-                for (int u = 0; u < kFpUnroll; ++u) consume(S[u]);
+                for (int c = 0; c < kFpCols; ++c)
+#pragma unroll
+                    for (int a = 0; a < LINES; ++a)
+                        t[c][a] = *reinterpret_cast<const PTaps*>(segs_bytes + lb[c] + a * nq16);
+#pragma unroll
+                for (int c = 0; c < kFpCols; ++c) {
+                    const float w[3] = {r[c].y, r[c].z, r[c].w};
+                    float sum = w[0] * fmaf(fr[c], t[c][0].b - t[c][0].a, t[c][0].a);
+#pragma unroll
+                    for (int a = 1; a < LINES; ++a)
+                        sum = fmaf(w[a], fmaf(fr[c], t[c][a].b - t[c][a].a, t[c][a].a), sum);
+                    H[c] += (double)sum;
+                }
+            };
+            // the rare walks (three segment loads, or three lines per column) go one slab at a
+            // time, which keeps their registers within the common walks' budget
+            constexpr bool kPair = NL <= 2 && LINES == 2;
+            while (mask) {
+                Set S0, S1;
+                const int q0 = __builtin_ctzll(mask) >> 2;
+                mask &= mask - 1;
+                issue(q0, S0);
+                if (kPair && mask) {
+                    const int q1 = __builtin_ctzll(mask) >> 2;
+                    mask &= mask - 1;
+                    issue(q1, S1);
+                    __builtin_amdgcn_sched_barrier(0);
+                    consume(S0);
+                    consume(S1);
+                } else {
+                    consume(S0);
+                }
             }
         };
-        const uint64_t m1 = __ballot(b_mode == 1);
-        if (m1) walk(std::integral_constant<int, 1>{}, m1);
-        const uint64_t m3 = __ballot(b_mode == 3);
-        if (m3) walk(std::integral_constant<int, 2>{}, m3);
-        // mode-2 slabs: explicit loop over every transverse cell of the footprint
-        uint64_t wide = __ballot(b_mode == 2);
-        while (wide) {
-            const int q = __builtin_ctzll(wide);
-            wide &= wide - 1;
-            const float tt = lane_f(b_tt, q);
-            unsigned off;
-            float fr;
-            edge(tt, off, fr);
-            const float alo = lane_f(b_alo, q), ahi = lane_f(b_ahi, q);
-            const float wa = ahi - alo;
-            const int kas = max((int)floorf(alo), 0), kae = min((int)floorf(ahi), nt - 1);
-            float sum = 0.0f;
-            for (int ka = kas; ka <= kae; ++ka) {
-                const float ov = fminf(ahi, (float)(ka + 1)) - fmaxf(alo, (float)ka);
-                if (ov < zero_tol * wa) continue;
-                const PTaps v = pair(off, q * slab_bytes + ka * line_bytes);
-                sum = fmaf(ov, fmaf(fr, v.b - v.a, v.a), sum);
+        const int wk = (u2 >> 24) & 7;
+        using I1 = std::integral_constant<int, 1>;
+        using I2 = std::integral_constant<int, 2>;
+        using I3 = std::integral_constant<int, 3>;
+        uint64_t m;
+        if ((m = __ballot(wk == 1) & heads)) walk(I1{}, I2{}, m);
+        if ((m = __ballot(wk == 2) & heads)) walk(I2{}, I2{}, m);
+        if ((m = __ballot(wk == 3) & heads)) walk(I3{}, I2{}, m);
+        if ((m = __ballot(wk == 4) & heads)) walk(I1{}, I3{}, m);
+        if ((m = __ballot(wk == 5) & heads)) walk(I2{}, I3{}, m);
+        if ((m = __ballot(wk == 6) & heads)) walk(I3{}, I3{}, m);
+        // explicit slabs: every transverse cell of each column's footprint, pairs read from
+        // memory
+        uint64_t m2 = __ballot(wk == 7) & heads;
+        while (m2) {
+            const int q = __builtin_ctzll(m2) >> 2;
+            m2 &= m2 - 1;
+            const float fn = (float)(n0 + q);
+#pragma unroll
+            for (int c = 0; c < kFpCols; ++c) {
+                const int pc = __builtin_amdgcn_readlane(p, 4 * q + c);
+                if (!(pc & kRecHit)) continue;
+                const float4 r = rec[4 * q + c];
+                unsigned off;
+                float fr;
+                edge(r.x, off, fr);
+                const float a1 = fmaf(lane_f(a_be, c), fn, lane_f(a_al, c));
+                const float a2 = fmaf(lane_f(b_be, c), fn, lane_f(b_al, c));
+                const float alo = fminf(a1, a2), ahi = fmaxf(a1, a2);
+                const float wa = ahi - alo;
+                const int kas = max((int)floorf(alo), 0), kae = min((int)floorf(ahi), nt - 1);
+                float sum = 0.0f;
+                for (int ka = kas; ka <= kae; ++ka) {
+                    const float ov = fminf(ahi, (float)(ka + 1)) - fmaxf(alo, (float)ka);
+                    if (ov < zero_tol * wa) continue;
+                    const v2u v = __builtin_amdgcn_raw_buffer_load_b64(rsrc, (int)off, q * slab_bytes + ka * line_bytes, 0);
+                    const float pa = __uint_as_float(v.x), pb = __uint_as_float(v.y);
+                    sum = fmaf(ov, fmaf(fr, pb - pa, pa), sum);
+                }
+                H[c] += (double)(sum * __builtin_amdgcn_rcpf(wa * r.x));
             }
-            H += (double)(sum * __builtin_amdgcn_rcpf(wa * tt));
         }
     }
-    // row k = edges k and k+1: the next lane's sum
-    const double Hn = __shfl_down(H, 1);
-    if (active) proj[out_idx] = (float)(Hn - H) * Lk;
+    // row k = edges k and k+1: the next lane's sum, times the ray's length factor
+    const float vc = rows[nv + 1 + kk];
+    const float dnu = rows[kk + 1] - rows[kk];
+    float res[kFpCols];
+#pragma unroll
+    for (int c = 0; c < kFpCols; ++c) {
+        const double Hn = __shfl_down(H[c], 1);
+        const int jj = min(j0 + c, nu - 1);
+        const float sec = T[4 * nu + 2 + jj], inv = T[5 * nu + 2 + jj];
+        const float s2 = vc * inv;
+        res[c] = (float)(Hn - H[c]) * (sec * sqrtf(1.0f + s2 * s2) / dnu);
+    }
+    if (!active) return;
+    if (vec_store) {
+        *reinterpret_cast<float4*>(out_row) = make_float4(res[0], res[1], res[2], res[3]);
+    } else {
+#pragma unroll
+        for (int c = 0; c < kFpCols; ++c)
+            if (j0 + c < nu) out_row[c] = res[c];
+    }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1660,13 +1769,13 @@ int ctr_dd_forward(const ctr_geometry* g, const double* angles, const int32_t* b
     hipLaunchKernelGGL(zprefix_kernel, dim3((g->nx + 31) / 32, g->ny), dim3(256), 0, s, vol, t_ys, t_xs,
                        g->nx, g->ny, g->nz, nzp);
     if ((st = status_from_hip(hipGetLastError()))) return st;
-    const int col_groups = (g->nu + kColWaves - 1) / kColWaves;
+    const int col_groups = (g->nu + kFpCols * kFpWaves - 1) / (kFpCols * kFpWaves);
     const int row_blocks = (g->nv + kColRows - 1) / kColRows;
     const int n_vblocks = (g->n_views + kViewBlock - 1) / kViewBlock;
     const long long n_blocks =
         8LL * kXcdRun * ((col_groups + 8 * kXcdRun - 1) / (8 * kXcdRun)) * row_blocks * n_vblocks * kViewBlock;
     if (n_blocks >= 2147483647LL) return CTR_ERR_INVALID;
-    hipLaunchKernelGGL(dd_forward_column_kernel, dim3((unsigned)n_blocks), dim3(256), 0, s, t_ys, t_xs,
+    hipLaunchKernelGGL(dd_forward_kernel, dim3((unsigned)n_blocks), dim3(64 * kFpWaves), 0, s, t_ys, t_xs,
                        proj, tabs, rows, branch, g->nx, g->ny, g->nz, nzp, g->nu, g->nv, stride, 1e-6f,
                        g->n_views, col_groups, n_vblocks);
     return status_from_hip(hipGetLastError());

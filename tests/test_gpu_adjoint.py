@@ -86,15 +86,22 @@ def test_adjoint_identity_at_batch_configuration(dev):
     assert rel <= 1e-6
 
 
-def test_finite_difference_gradient_of_data_term(dev):
-    # f(x) = ||A x - b||^2 ; autograd gradient (backward = A^T) vs central differences.  f is
-    # quadratic in x, so a central difference is exact at any step; a large step keeps the FP32
-    # rounding of the projections (relative ~1e-7 of the z line integrals) far below it
+@pytest.mark.parametrize("eps,tol_max,tol_median", [(0.25, 5e-3, 1e-4), (1e-3, 5e-2, 1e-3)])
+def test_finite_difference_gradient_of_data_term(dev, eps, tol_max, tol_median):
+    """f(x) = ||A x - b||^2 ; autograd gradient (backward = A^T) vs central differences on 64
+    voxels of the 128^3 / 180-view / 256^2 configuration.  f is quadratic in x, so a central
+    difference is exact at any step up to rounding: at eps = 0.25 it pins the gradient at
+    5e-3 of its size.  At the configured eps = 1e-3 the FP32 rounding of the forward projector
+    shows: a z-prefix breakpoint holds a line integral to half an ulp, and a changed voxel
+    re-rounds every breakpoint above it, so f moves by rounding noise beside the 2 eps g signal
+    (measured: median 1.5e-4 to 3.5e-4, worst of 64 voxels 7e-3 to 2.8e-2 of the gradient).  A
+    wrong gradient would be off by O(1)."""
     geom = ctr.ScanGeometry(make_params(**C4))
     ell = phantom.random_ellipsoids(8, phantom.half_extent(geom), 401)
     x = (phantom.ellipsoid_volume(geom, ell, device=dev) * 0.5).contiguous()
-    b = ctr.dd_forward(x, geom) + 0.1 * torch.randn(geom.proj_shape, device=dev)
-    x0 = (x + 0.05 * torch.randn_like(x)).contiguous()
+    gen = torch.Generator(device=dev).manual_seed(403)
+    b = ctr.dd_forward(x, geom) + 0.1 * torch.randn(geom.proj_shape, device=dev, generator=gen)
+    x0 = (x + 0.05 * torch.randn(x.shape, device=dev, generator=gen)).contiguous()
     xv = x0.clone().requires_grad_(True)
     f = torch.sum((DDProjectorFunction.apply(xv, geom) - b).double() ** 2)
     (grad,) = torch.autograd.grad(f, xv)
@@ -105,7 +112,6 @@ def test_finite_difference_gradient_of_data_term(dev):
 
     rng = np.random.default_rng(402)
     idx = [tuple(int(rng.integers(16, n - 16)) for n in geom.vol_shape) for _ in range(64)]
-    eps = 0.25
     num, ana = [], []
     for i in idx:
         xp = x0.clone()
@@ -116,8 +122,8 @@ def test_finite_difference_gradient_of_data_term(dev):
         ana.append(float(grad[i]))
     num, ana = np.array(num), np.array(ana)
     err = np.abs(num - ana) / np.maximum(np.abs(ana), 1e-3 * np.abs(ana).max())
-    print(f"finite-difference check: max rel err {err.max():.2e}, median {np.median(err):.2e}")
-    assert err.max() <= 5e-3
+    print(f"finite-difference check, eps {eps}: max rel err {err.max():.2e}, median {np.median(err):.2e}")
+    assert err.max() <= tol_max and np.median(err) <= tol_median
 
 
 def test_autograd_backward_is_matched_backprojector(dev):
@@ -188,9 +194,11 @@ def _tv(x):
 
 
 def test_asd_pocs_tv_steps_through_autograd(dev):
-    geom = ctr.ScanGeometry(make_params(**dict(C4, NumberOfViews=60)))
+    """ASD-POCS (python/ASD_POCS.py:167-199) on the batch configuration: 8 volumes of 128^3,
+    180 views, 256^2 cells; 20 TV steps per outer iteration, the TV gradient by autograd."""
+    geom = ctr.ScanGeometry(make_params(**C4))
     vols = torch.stack([torch.clamp_min(phantom.ellipsoid_volume(geom, phantom.random_ellipsoids(
-        8, phantom.half_extent(geom), 420 + b), device=dev), 0.0) for b in range(2)])
+        8, phantom.half_extent(geom), 420 + b), device=dev), 0.0) for b in range(8)])
     from ctreconstruction_amd.autograd import forward_batch
     proj = forward_batch(vols, geom)
     kw = dict(na=5, backprojector="matched")
