@@ -38,6 +38,7 @@ HBM_PEAK_GBS = 8000.0
 # launches a flag fill and the remainder pass, both under 1% of it for this workload; the HIP
 # events bracket all three, so the achieved rate is a lower bound for the pass itself)
 BP_KERNEL = "dd_backproject_tile_kernel<16, 0>"
+FP_KERNEL = "dd_forward_kernel"
 
 # The reference's own geometry file for its pixel-driven FDK (python/ReconstructionParams.txt:1-11,
 # read by python/ConeBeamGPU.py:164-197); AngleCoverage in radians as the reader stores it.
@@ -155,15 +156,27 @@ def cpu_baseline(params, q_gpu, geom, dev, pool=None, sample_views=(0, 1), slab=
     if pool is not None:
         workers = pool._processes
         mslab = min(16, g.nz)
-        step = max(g.nview // (2 * workers), 1)
-        sample = [list(range(w * 2 * step, w * 2 * step + 2 * step, step)) for w in range(workers)]
         raw = q_gpu["raw"]
-        jobs = [(params, v, raw[v].cpu().numpy(), mslab) for v in sample]
-        t0 = time.perf_counter()
-        res = pool.map(_oracle_views, jobs)
-        wall = time.perf_counter() - t0
-        n_done = sum(len(v) for v in sample)
-        full = wall * (g.nview / n_done) * (g.nz / mslab)
+
+        def run(per_worker):
+            # per_worker views per process, spread over the scan
+            step = max(g.nview // (per_worker * workers), 1)
+            sample = [list(range(w * per_worker * step, (w + 1) * per_worker * step, step)) for w in range(workers)]
+            jobs = [(params, v, raw[v].cpu().numpy(), mslab) for v in sample]
+            t0 = time.perf_counter()
+            res = pool.map(_oracle_views, jobs)
+            wall = time.perf_counter() - t0
+            n_done = sum(len(v) for v in sample)
+            return res, wall, n_done, wall * (g.nview / n_done) * (g.nz / mslab)
+
+        # linearity of the extrapolation: one view per process, then two
+        _, wall1, n1, full1 = run(1)
+        res, wall, n_done, full = run(2)
+        out["extrapolation_check"] = {
+            "one_view_per_process_s": full1, "two_views_per_process_s": full,
+            "ratio": full / full1,
+            "note": (f"{n1} views in {wall1:.2f} s wall and {n_done} views in {wall:.2f} s wall, each "
+                     f"scaled linearly to the full reconstruction")}
         out.update({
             "value": full, "cores": workers,
             "sample": (f"FP64 NumPy oracle on {workers} processes (one per host core): filter and "
@@ -255,6 +268,49 @@ def pixel_fdk_leg(dev, stream, steps, warmup, cpu=True, seed=3):
     return out
 
 
+CALIB_KINDS = ("v_fma_f32", "v_add_u32", "v_cndmask_b32", "v_readfirstlane_b32", "v_rcp_f32", "v_add_f64",
+               "v_mul_u32_u24")
+SIMDS = 1024          # 256 CUs x 4 SIMDs
+NOMINAL_CLOCK_HZ = 2.4e9
+
+
+def valu_calibration(dev, stream, blocks=4096, iters=4096, reps=3):
+    """Issue-rate microbenchmarks (ctreconstruction_amd/csrc/calibrate.hip): for each instruction
+    kind, kernels whose wavefronts issue exactly 8 * iters instructions of that kind, event-timed.
+    The rate is wave-instructions per SIMD-cycle at the nominal 2.4 GHz clock; a rocprofv3 pass over the same run reads SQ_INSTS_VALU and
+    GRBM_GUI_ACTIVE for the measured clock."""
+    import ctypes
+
+    import torch
+
+    from ctreconstruction_amd import _native as N
+
+    lib = ctypes.CDLL(N.CALIB_PATH)
+    lib.ctr_calib_run.restype = ctypes.c_int
+    lib.ctr_calib_run.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    sink = torch.empty(blocks * 256, dtype=torch.float32, device=dev)
+    st = ctypes.c_void_p(stream.cuda_stream)
+    out = {"blocks": blocks, "threads_per_block": 256, "iters": iters, "clock_hz_assumed": NOMINAL_CLOCK_HZ}
+    waves = blocks * 4
+    for kind, name in enumerate(CALIB_KINDS):
+        per_iter = 8
+        if lib.ctr_calib_run(kind, blocks, iters, ctypes.c_void_p(sink.data_ptr()), st) != 0:
+            raise RuntimeError(f"calibration kernel {name} failed to launch")
+        ts = []
+        for _ in range(reps):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            lib.ctr_calib_run(kind, blocks, iters, ctypes.c_void_p(sink.data_ptr()), st)
+            b.record(stream)
+            torch.cuda.synchronize()
+            ts.append(a.elapsed_time(b) / 1e3)
+        t = float(np.median(ts))
+        wave_instr = float(waves) * iters * per_iter
+        out[name] = {"s": t, "wave_instructions": wave_instr,
+                     "per_simd_cycle": wave_instr / (SIMDS * NOMINAL_CLOCK_HZ * t)}
+    return out
+
+
 def learned_vs_sart(seed=10000):
     """Sparse-view benchmark (256^3, 60 views, 384^2, Poisson 1e5, held-out phantom): wall time
     and RMSE vs the phantom of plain 200-iteration SART and of the learned prior + SART pipeline
@@ -280,6 +336,8 @@ def main(argv=None):
     ap.add_argument("--no-pixel", action="store_true", help="skip the pixel-driven FDK leg")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (rocprof counter calibration)")
+    ap.add_argument("--calibrate-valu", action="store_true",
+                    help="also run the instruction-issue microbenchmarks (VALU ceiling calibration)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the view-sharded multi-GPU layout is "
@@ -312,7 +370,7 @@ def main(argv=None):
     proj = ctr.dd_forward(vol, geom)
     torch.cuda.synchronize()
     fp_times = []
-    for _ in range(2):
+    for _ in range(5):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
         ctr.dd_forward(vol, geom, out=proj)
@@ -357,6 +415,8 @@ def main(argv=None):
         b = torch.empty_like(a)
         b.copy_(a)
         torch.cuda.synchronize()
+        del a, b
+    calib = valu_calibration(dev, stream) if args.calibrate_valu else None
 
     nvox = geom.nx * geom.ny * geom.nz
     ncell = geom.nview * geom.nv * geom.nu
@@ -366,7 +426,7 @@ def main(argv=None):
     fp_s = float(np.median(fp_times))
     ray_slab = float(ncell) * geom.ny  # every ray crosses n_slab slabs (ny == nx here)
     bp_prof = profile_record(f"dd_backproject_{args.config}_traffic", BP_KERNEL)
-    fp_prof = profile_record(f"dd_forward_{args.config}_traffic", "dd_forward_column_kernel")
+    fp_prof = profile_record(f"dd_forward_{args.config}_traffic", FP_KERNEL)
     out = {
         "metric": "FDK 512³/720-view recon s; DD proj GUPS; SART vs learned time-to-RMSE",
         "value": sec, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -396,6 +456,8 @@ def main(argv=None):
                                                              cpu=not args.no_cpu_baseline),
         "setup_s": t_setup,
     }
+    if calib is not None:
+        out["valu_calibration"] = calib
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(params, {"raw": proj}, geom, dev, pool)
     if pool is not None:

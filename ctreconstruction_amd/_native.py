@@ -19,6 +19,9 @@ LIB_PATH = os.path.join(LIB_DIR, "libctrecon.so")
 SOURCES = [os.path.join(PKG_DIR, "csrc", "ctrecon.hip"), os.path.join(PKG_DIR, "csrc", "prior.hip"),
            os.path.join(PKG_DIR, "csrc", "pixel.hip")]
 INCLUDE_DIR = os.path.join(REPO_DIR, "include")
+# instruction-issue microbenchmarks for bench.py --calibrate-valu (not part of the product ABI)
+CALIB_PATH = os.path.join(LIB_DIR, "libctrcalib.so")
+CALIB_SOURCES = [os.path.join(PKG_DIR, "csrc", "calibrate.hip")]
 
 # Exported symbols (every function declared in include/ctrecon.h).
 EXPORTS = (
@@ -74,27 +77,33 @@ class CtrError(RuntimeError):
 _lib = None
 
 
-def hipcc_command(out: str = LIB_PATH) -> list[str]:
+def hipcc_command(out: str = LIB_PATH, sources=None) -> list[str]:
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
     return [hipcc, "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             # device code never produces or tests NaNs: lets min/max skip NaN quieting
             "-Xarch_device", "-fno-honor-nans",
-            "-I", INCLUDE_DIR, "-o", out, *SOURCES]
+            "-I", INCLUDE_DIR, "-o", out, *(SOURCES if sources is None else sources)]
 
 
-def build(verbose: bool = False) -> str:
-    """Compile the HIP library for gfx950 into ``lib/libctrecon.so`` (always recompiles)."""
-    os.makedirs(LIB_DIR, exist_ok=True)
-    tmp = LIB_PATH + ".tmp"
-    cmd = hipcc_command(tmp)
+def _compile(out: str, sources, verbose: bool) -> str:
+    tmp = out + ".tmp"
+    cmd = hipcc_command(tmp, sources)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + " ".join(cmd) + "\n" + res.stdout + res.stderr)
     if verbose:
         print(" ".join(cmd))
-    os.replace(tmp, LIB_PATH)
-    return LIB_PATH
+    os.replace(tmp, out)
+    return out
+
+
+def build(verbose: bool = False) -> str:
+    """Compile the HIP library for gfx950 into ``lib/libctrecon.so`` (always recompiles), and the
+    calibration microbenchmarks into ``lib/libctrcalib.so``."""
+    os.makedirs(LIB_DIR, exist_ok=True)
+    _compile(CALIB_PATH, CALIB_SOURCES, verbose)
+    return _compile(LIB_PATH, SOURCES, verbose)
 
 
 def _declare(lib):

@@ -808,33 +808,186 @@ __device__ inline int window_mode(int wvn, int wun) {
 // the rare cases out of the fast pass keeps its register count, and so its occupancy, low.
 constexpr int kBpFast = 0, kBpAll = 1, kBpRest = 2;
 
-// Returns true when KIND == kBpFast leaves this (work-item, view) to the completing pass.
-template <int ZC, bool FROM_LDS, int KIND>
-__device__ inline bool bp_column_view(const BpView& v, const float* __restrict__ src, int stride,
-                                      int row0, int nrows, int col0, int ncols, float X, float Y,
-                                      float Z0, float dz, float hx, float hy, float hz, int nz_left,
-                                      int curved, float scale, float zero_tol, int nu, int nv,
-                                      bool window_in_lds, float* __restrict__ scan, float (&acc)[ZC]) {
+// Footprint of a voxel column for one view: the four corners (+-hx, +-hy) rotated into the
+// source frame give the u-footprint [ulo, uhi] (index space) and the range [rmin, rmax] of
+// 1/(Sy - y') over the corners.  Every pass computes it here, so all passes round it alike.
+struct BpColumn {
+    float yr0, rmin, rmax, ulo, uhi;
+    int ku0, ku1;
+};
+
+__device__ inline BpColumn bp_column(const BpView& v, float X, float Y, float hx, float hy, int curved) {
+    BpColumn col;
     const float xr0 = fmaf(X, v.c, Y * v.s);
-    const float yr0 = fmaf(Y, v.c, -(X * v.s));
+    col.yr0 = fmaf(Y, v.c, -(X * v.s));
     // the corners (+-hx, +-hy) rotate to (xr0, yr0) +- (P, M) and (xr0, yr0) +- (Q, -N)
     const float P = fmaf(hx, v.c, hy * v.s), Q = fmaf(hx, v.c, -(hy * v.s));
     const float M = fmaf(hy, v.c, -(hx * v.s)), N = fmaf(hy, v.c, hx * v.s);
     // corner first, then its distance from the source: one rounding at the source's scale
-    const float r0 = __builtin_amdgcn_rcpf(v.Sy - (yr0 + M)), r1 = __builtin_amdgcn_rcpf(v.Sy - (yr0 - M));
-    const float r2 = __builtin_amdgcn_rcpf(v.Sy - (yr0 - N)), r3 = __builtin_amdgcn_rcpf(v.Sy - (yr0 + N));
+    const float r0 = __builtin_amdgcn_rcpf(v.Sy - (col.yr0 + M)), r1 = __builtin_amdgcn_rcpf(v.Sy - (col.yr0 - M));
+    const float r2 = __builtin_amdgcn_rcpf(v.Sy - (col.yr0 - N)), r3 = __builtin_amdgcn_rcpf(v.Sy - (col.yr0 + N));
     const float su0 = (v.Sx - (xr0 + P)) * r0, su1 = (v.Sx - (xr0 - P)) * r1;
     const float su2 = (v.Sx - (xr0 + Q)) * r2, su3 = (v.Sx - (xr0 - Q)) * r3;
     const float smin = fminf(fminf(su0, su1), fminf(su2, su3));
     const float smax = fmaxf(fmaxf(su0, su1), fmaxf(su2, su3));
-    const float rmin = fminf(fminf(r0, r1), fminf(r2, r3));
-    const float rmax = fmaxf(fmaxf(r0, r1), fmaxf(r2, r3));
+    col.rmin = fminf(fminf(r0, r1), fminf(r2, r3));
+    col.rmax = fmaxf(fmaxf(r0, r1), fmaxf(r2, r3));
     const float fu1 = u_index(v, smin, curved), fu2 = u_index(v, smax, curved);
-    const float ulo = fminf(fu1, fu2), uhi = fmaxf(fu1, fu2);
-    const int ku0 = (int)floorf(ulo), ku1 = (int)floorf(uhi);
+    col.ulo = fminf(fu1, fu2);
+    col.uhi = fmaxf(fu1, fu2);
+    col.ku0 = (int)floorf(col.ulo);
+    col.ku1 = (int)floorf(col.uhi);
+    return col;
+}
+
+// v-footprints (index space) of the ZC voxels of a chunk.  Voxel i has e = Sz - z = e0 - i dz and
+// its footprint ends are g1 = Av - Bv (e - hz) r1, g2 = Av - Bv (e + hz) r2 with r1 = rmin,
+// r2 = rmax where e -/+ hz >= 0 and r1 = rmax, r2 = rmin where e -/+ hz <= 0.  When the whole
+// chunk lies on one side of the source height (uniform across the workgroup: `lin`) both ends
+// are affine in i, lo = Alo + i Dlo and hi = Ahi + i Dhi, and g2 - g1 = -Bv (smx - smn) has the
+// sign of -Bv for every voxel.
+struct BpChunk {
+    float Alo, Dlo, Ahi, Dhi;
+    bool lin;
+};
+
+template <int ZC>
+__device__ inline BpChunk bp_chunk(const BpView& v, const BpColumn& col, float Z0, float dz, float hz) {
+    BpChunk ch;
+    const float e0 = v.Sz - Z0;
+    const float eL = fmaf(-(float)(ZC - 1), dz, e0);
+    const bool above = fminf(e0, eL) - hz >= 0.0f;
+    ch.lin = above || (fmaxf(e0, eL) + hz <= 0.0f);
+    const float r1 = above ? col.rmin : col.rmax, r2 = above ? col.rmax : col.rmin;
+    const float B1 = v.Bv * r1, B2 = v.Bv * r2;
+    const float A1 = fmaf(-B1, e0 - hz, v.Av), D1 = B1 * dz;
+    const float A2 = fmaf(-B2, e0 + hz, v.Av), D2 = B2 * dz;
+    const bool g1_high = v.Bv > 0.0f;
+    ch.Alo = g1_high ? A2 : A1;
+    ch.Dlo = g1_high ? D2 : D1;
+    ch.Ahi = g1_high ? A1 : A2;
+    ch.Dhi = g1_high ? D1 : D2;
+    return ch;
+}
+
+// Row integral C at row boundaries ks .. ks+nr of the window rows from `rp` (stride `stride`),
+// u-filtered with three taps (weight 0 past the footprint; the window buffers are zero-
+// initialised and padded, so such reads stay finite) and times W, in this work-item's LDS
+// column `scan` (stride 256), plus a copy of the total at ks+nr+1 so that a lookup at the top
+// needs no index clamp.  Two rows per step: both rows' loads go out before one wait.
+__device__ inline void bp_row_integral3(const float* __restrict__ rp, int stride, int nr, const float (&wu)[kMaxUTaps],
+                                        float W, float* __restrict__ scan) {
+    float csum = 0.0f;
+    scan[0] = 0.0f;
+    float* sp = scan;
+    int j = 0;
+    for (; j + 1 < nr; j += 2) {
+        const float* rq = rp + stride;
+        const float rk0 = fmaf(wu[0], rp[0], fmaf(wu[1], rp[1], wu[2] * rp[2]));
+        const float rk1 = fmaf(wu[0], rq[0], fmaf(wu[1], rq[1], wu[2] * rq[2]));
+        csum = fmaf(rk0, W, csum);
+        sp[256] = csum;
+        csum = fmaf(rk1, W, csum);
+        sp[512] = csum;
+        sp += 512;
+        rp += 2 * stride;
+    }
+    if (j < nr) {
+        const float rk = fmaf(wu[0], rp[0], fmaf(wu[1], rp[1], wu[2] * rp[2]));
+        csum = fmaf(rk, W, csum);
+        sp[256] = csum;
+    }
+    scan[(nr + 1) * 256] = csum;
+}
+
+// The voxels of a chunk with affine footprints from the row integral: C between row boundaries
+// is linear, and clamping v - ks to [0, nr] gives C = 0 below and the total above (rows outside
+// the detector or the window contribute nothing).  1/(hi - lo) of voxel i comes by one Newton
+// step from voxel i-1's: the footprint height is affine in i and changes by a small fraction d
+// per voxel (the magnification spread over one voxel), so the step leaves a relative error of
+// about d^2 (below 1e-8 at 512^3) and costs two FMAs instead of a reciprocal, which issues at
+// half the FMA rate.
+template <int ZC>
+__device__ inline void bp_voxels_affine(const BpChunk& ch, int ks, int nr, const float* __restrict__ scan,
+                                        float (&acc)[ZC]) {
+    const float fks = (float)ks, fnr = (float)nr;
+    auto integral = [&](float x) {
+        const float f = __builtin_amdgcn_fmed3f(x, 0.0f, fnr);
+        const int jj = (int)f;
+        const float a = scan[jj * 256], b = scan[(jj + 1) * 256];
+        return fmaf(__builtin_amdgcn_fractf(f), b - a, a);
+    };
+    const float alo = ch.Alo - fks, ahi = ch.Ahi - fks;
+    float rinv = __builtin_amdgcn_rcpf(ahi - alo);
+#pragma unroll
+    for (int i = 0; i < ZC; ++i) {
+        const float lo = fmaf((float)i, ch.Dlo, alo), hi = fmaf((float)i, ch.Dhi, ahi);
+        if (i > 0) rinv *= fmaf(lo - hi, rinv, 2.0f);
+        acc[i] = fmaf(integral(hi) - integral(lo), rinv, acc[i]);
+    }
+}
+
+// The fast pass (kBpFast): the common case only -- window in LDS, chunk on one side of the
+// source height, at most kMaxRows rows and kMaxUTaps columns, no weight zeroing (the host runs
+// this pass only with zero_tol = 0).  Returns true when it leaves the (work-item, view) to the
+// completing pass, which decides from the same bp_column / bp_chunk values.
+template <int ZC>
+__device__ inline bool bp_column_view_fast(const BpView& v, const float* __restrict__ win, int stride,
+                                           int row0, int nrows, int col0, int ncols, float X, float Y,
+                                           float Z0, float dz, float hx, float hy, float hz, int nz_left,
+                                           int curved, float scale, int nu, int nv, float* __restrict__ scan,
+                                           float (&acc)[ZC]) {
+    const BpColumn col = bp_column(v, X, Y, hx, hy, curved);
+    // taps inside the detector and the window
+    const int kus = max(max(col.ku0, 0), col0), kue = min(min(col.ku1, nu - 1), col0 + ncols - 1);
+    if (kus > kue) return false;
+    const BpChunk ch = bp_chunk<ZC>(v, col, Z0, dz, hz);
+    if (!ch.lin) return true;
+    // the ends move monotonically with i, so the chunk's row range comes from its end voxels
+    const float fL = (float)(nz_left - 1);
+    const float lo0 = fmaf(0.0f, ch.Dlo, ch.Alo), hi0 = fmaf(0.0f, ch.Dhi, ch.Ahi);
+    const float loL = fmaf(fL, ch.Dlo, ch.Alo), hiL = fmaf(fL, ch.Dhi, ch.Ahi);
+    const float rlo = fminf(lo0, loL), rhi = fmaxf(hi0, hiL);
+    const int ks = max(max((int)floorf(rlo), 0), row0);
+    const int ke = min(min((int)floorf(rhi), nv - 1), row0 + nrows - 1);
+    const int nr = ke - ks + 1;
+    if (nr <= 0) return false;
+    if (nr > kMaxRows || kue - kus + 1 > kMaxUTaps) return true;
+    // overlap of each tap with [ulo, uhi] over the footprint width (python/GPUFuncs_pycuda.py:
+    // 427-438 without zeroing); a footprint inside one cell weighs 1
+    const float inv_u = __builtin_amdgcn_rcpf(col.uhi - col.ulo);
+    const float fk = (float)kus;
+    float wu[kMaxUTaps];
+#pragma unroll
+    for (int j = 0; j < kMaxUTaps; ++j) {
+        const float k = fk + (float)j;
+        const float w = (fminf(col.uhi, k + 1.0f) - fmaxf(col.ulo, k)) * inv_u;
+        wu[j] = (kus + j <= kue) ? fmaxf(w, 0.0f) : 0.0f;
+    }
+    if (col.ku0 == col.ku1) wu[0] = 1.0f;
+    const float dd = v.R - col.yr0;
+    const float W = scale * (v.R * v.R) * __builtin_amdgcn_rcpf(dd * dd);
+    // window offsets fit 24 bits: a full-rate multiply (the 32-bit one is quarter rate)
+    bp_row_integral3(win + __mul24(ks - row0, stride) + (kus - col0), stride, nr, wu, W, scan);
+    bp_voxels_affine<ZC>(ch, ks, nr, scan, acc);
+    return false;
+}
+
+// The completing pass (kBpRest: every (work-item, view) the fast pass left, and nothing else)
+// and the single pass over all cases (kBpAll).
+template <int ZC, bool FROM_LDS, int KIND>
+__device__ inline void bp_column_view(const BpView& v, const float* __restrict__ src, int stride,
+                                      int row0, int nrows, int col0, int ncols, float X, float Y,
+                                      float Z0, float dz, float hx, float hy, float hz, int nz_left,
+                                      int curved, float scale, float zero_tol, int nu, int nv,
+                                      bool window_in_lds, float* __restrict__ scan, float (&acc)[ZC]) {
+    static_assert(KIND == kBpAll || KIND == kBpRest, "the fast pass has its own function");
+    const BpColumn col = bp_column(v, X, Y, hx, hy, curved);
+    const float ulo = col.ulo, uhi = col.uhi;
+    const int ku0 = col.ku0, ku1 = col.ku1;
     // taps inside the detector and the source window
     const int kus = max(max(ku0, 0), col0), kue = min(min(ku1, nu - 1), col0 + ncols - 1);
-    if (kus > kue) return false;
+    if (kus > kue) return;
     const float inv_u = __builtin_amdgcn_rcpf(uhi - ulo);
     float wu[kMaxUTaps];
 #pragma unroll
@@ -842,33 +995,15 @@ __device__ inline bool bp_column_view(const BpView& v, const float* __restrict__
         const int ku = kus + j;
         wu[j] = (ku <= kue) ? dd_weight(ku, ku0, ku1, ulo, uhi, inv_u, zero_tol) : 0.0f;
     }
-    const float dd = v.R - yr0;
+    const float dd = v.R - col.yr0;
     const float W = scale * (v.R * v.R) * __builtin_amdgcn_rcpf(dd * dd);
-    // v-footprints (index space) of the ZC voxels.  Voxel i has e = Sz - z = e0 - i dz and its
-    // footprint ends are g1 = Av - Bv (e - hz) r1, g2 = Av - Bv (e + hz) r2 with r1 = rmin,
-    // r2 = rmax where e -/+ hz >= 0 and r1 = rmax, r2 = rmin where e -/+ hz <= 0.  When the whole
-    // chunk lies on one side of the source height (uniform across the workgroup) both ends are
-    // affine in i, and g2 - g1 = -Bv (smx - smn) has the sign of -Bv for every voxel.
-    const float e0 = v.Sz - Z0;
-    const float eL = fmaf(-(float)(ZC - 1), dz, e0);
-    const bool above = fminf(e0, eL) - hz >= 0.0f;
-    const bool lin = above || (fmaxf(e0, eL) + hz <= 0.0f);
-    float Alo, Dlo, Ahi, Dhi;
-    {
-        const float r1 = above ? rmin : rmax, r2 = above ? rmax : rmin;
-        const float B1 = v.Bv * r1, B2 = v.Bv * r2;
-        const float A1 = fmaf(-B1, e0 - hz, v.Av), D1 = B1 * dz;
-        const float A2 = fmaf(-B2, e0 + hz, v.Av), D2 = B2 * dz;
-        const bool g1_high = v.Bv > 0.0f;
-        Alo = g1_high ? A2 : A1;
-        Dlo = g1_high ? D2 : D1;
-        Ahi = g1_high ? A1 : A2;
-        Dhi = g1_high ? D1 : D2;
-    }
+    const BpChunk ch = bp_chunk<ZC>(v, col, Z0, dz, hz);
+    const bool lin = ch.lin;
+    const float rmin = col.rmin, rmax = col.rmax;
     auto foot = [&](float fi, float& lo, float& hi) {
         if (lin) {
-            lo = fmaf(fi, Dlo, Alo);
-            hi = fmaf(fi, Dhi, Ahi);
+            lo = fmaf(fi, ch.Dlo, ch.Alo);
+            hi = fmaf(fi, ch.Dhi, ch.Ahi);
         } else {
             const float e = v.Sz - fmaf(fi, dz, Z0);
             const float ep = e + hz, em = e - hz;
@@ -887,48 +1022,21 @@ __device__ inline bool bp_column_view(const BpView& v, const float* __restrict__
     const int ks = max(max((int)floorf(rlo), 0), row0);
     const int ke = min(min((int)floorf(rhi), nv - 1), row0 + nrows - 1);
     const int nr = ke - ks + 1;
-    if (nr <= 0) return false;
+    if (nr <= 0) return;
     const int cbase = kus - col0;
     const int ntaps = kue - kus + 1;
-    // the cases the fast pass does
-    const bool common = window_in_lds && lin && nr <= kMaxRows && ntaps <= kMaxUTaps && zero_tol <= 0.0f;
-    if constexpr (KIND == kBpFast) {
-        if (!common) return true;
-    }
     if constexpr (KIND == kBpRest) {
-        if (common) return false;
+        // the pairs the fast pass did
+        const bool common = window_in_lds && lin && nr <= kMaxRows && ntaps <= kMaxUTaps && zero_tol <= 0.0f;
+        if (common) return;
     }
-    if (KIND == kBpFast || (zero_tol <= 0.0f && nr <= kMaxRows)) {
-        // row integral C at row boundaries ks .. ks+nr in this work-item's LDS column, plus a
-        // copy of the total at ks+nr+1 so that a lookup at the top needs no index clamp
-        float csum = 0.0f;
-        scan[0] = 0.0f;
-        // window offsets fit 24 bits: a full-rate multiply (the 32-bit one is quarter rate)
+    if (zero_tol <= 0.0f && nr <= kMaxRows) {
         const float* rp = src + (FROM_LDS ? __mul24(ks - row0, stride) : (size_t)(ks - row0) * stride) + cbase;
-        if (KIND == kBpFast || (FROM_LDS && !__any(ntaps > kMaxUTaps))) {
-            // the three taps are consecutive window cells (weight 0 past the footprint; the
-            // window buffers are zero-initialised and padded, so such reads stay finite)
-            float* sp = scan;
-            // two rows per step: both rows' loads go out before one wait, and the loop's own
-            // work is shared; an odd last row follows
-            int j = 0;
-            for (; j + 1 < nr; j += 2) {
-                const float* rq = rp + stride;
-                const float rk0 = fmaf(wu[0], rp[0], fmaf(wu[1], rp[1], wu[2] * rp[2]));
-                const float rk1 = fmaf(wu[0], rq[0], fmaf(wu[1], rq[1], wu[2] * rq[2]));
-                csum = fmaf(rk0, W, csum);
-                sp[256] = csum;
-                csum = fmaf(rk1, W, csum);
-                sp[512] = csum;
-                sp += 512;
-                rp += 2 * stride;
-            }
-            if (j < nr) {
-                const float rk = fmaf(wu[0], rp[0], fmaf(wu[1], rp[1], wu[2] * rp[2]));
-                csum = fmaf(rk, W, csum);
-                sp[256] = csum;
-            }
+        if (FROM_LDS && !__any(ntaps > kMaxUTaps)) {
+            bp_row_integral3(rp, stride, nr, wu, W, scan);
         } else {
+            float csum = 0.0f;
+            scan[0] = 0.0f;
             for (int j = 0; j < nr; ++j) {
                 float rk = 0.0f;
 #pragma unroll
@@ -939,31 +1047,18 @@ __device__ inline bool bp_column_view(const BpView& v, const float* __restrict__
                 scan[(j + 1) * 256] = csum;
                 rp += stride;
             }
+            scan[(nr + 1) * 256] = csum;
         }
-        scan[(nr + 1) * 256] = csum;
-        // C(v) between row boundaries is linear; clamping v - ks to [0, nr] gives C = 0 below
-        // and the total above (rows outside the detector or the window contribute nothing)
-        const float fks = (float)ks, fnr = (float)nr;
-        auto integral = [&](float x) {
-            const float f = __builtin_amdgcn_fmed3f(x, 0.0f, fnr);
-            const int jj = (int)f;
-            const float a = scan[jj * 256], b = scan[(jj + 1) * 256];
-            return fmaf(__builtin_amdgcn_fractf(f), b - a, a);
-        };
-        if (KIND == kBpFast || lin) {
-            const float alo = Alo - fks, ahi = Ahi - fks;
-            // 1/(hi - lo) of voxel i by one Newton step from voxel i-1's: the footprint height is
-            // affine in i and changes by a small fraction d per voxel (the magnification spread
-            // over one voxel), so the step leaves a relative error of about d^2 (below 1e-8 at
-            // 512^3) and costs two FMAs instead of a reciprocal, which issues at half the FMA rate
-            float rinv = __builtin_amdgcn_rcpf(ahi - alo);
-#pragma unroll
-            for (int i = 0; i < ZC; ++i) {
-                const float lo = fmaf((float)i, Dlo, alo), hi = fmaf((float)i, Dhi, ahi);
-                if (i > 0) rinv *= fmaf(lo - hi, rinv, 2.0f);
-                acc[i] = fmaf(integral(hi) - integral(lo), rinv, acc[i]);
-            }
+        if (lin) {
+            bp_voxels_affine<ZC>(ch, ks, nr, scan, acc);
         } else {
+            const float fks = (float)ks, fnr = (float)nr;
+            auto integral = [&](float x) {
+                const float f = __builtin_amdgcn_fmed3f(x, 0.0f, fnr);
+                const int jj = (int)f;
+                const float a = scan[jj * 256], b = scan[(jj + 1) * 256];
+                return fmaf(__builtin_amdgcn_fractf(f), b - a, a);
+            };
 #pragma unroll
             for (int i = 0; i < ZC; ++i) {
                 float lo, hi;
@@ -973,7 +1068,7 @@ __device__ inline bool bp_column_view(const BpView& v, const float* __restrict__
                 acc[i] = fmaf(integral(hi) - integral(lo), __builtin_amdgcn_rcpf(hi - lo), acc[i]);
             }
         }
-        return false;
+        return;
     }
     // explicit per-row weights (zeroing of small weights, or a chunk over more than kMaxRows
     // rows): rare, so the voxel footprints are recomputed per row rather than held in registers
@@ -995,7 +1090,6 @@ __device__ inline bool bp_column_view(const BpView& v, const float* __restrict__
             acc[i] = fmaf(w, rk, acc[i]);
         }
     }
-    return false;
 }
 
 // KIND: kBpFast writes the volume and sets flags[tile][view] for every view it leaves (in
@@ -1146,9 +1240,8 @@ __global__ void __launch_bounds__(256) dd_backproject_tile_kernel(
                     continue;
                 }
                 if (!active) continue;
-                if (bp_column_view<ZC, true, kBpFast>(load_view(vt, vw), win[k], wun[k], w0[k], wn[k],
-                                                      wu[k], wun[k], X, Y, Z0, dz, hx, hy, hz, nz_left, curved,
-                                                      scale, zero_tol, nu, nv, true, scan, acc))
+                if (bp_column_view_fast<ZC>(load_view(vt, vw), win[k], wun[k], w0[k], wn[k], wu[k], wun[k], X, Y,
+                                            Z0, dz, hx, hy, hz, nz_left, curved, scale, nu, nv, scan, acc))
                     tflags[vw] = 1;
             } else {
                 if (!active) continue;
