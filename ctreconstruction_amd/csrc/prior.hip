@@ -10,9 +10,10 @@
 //   conv3x3_fwd_persistent_kernel  the same for 64 -> 64 channels: one workgroup per CU with
 //                         all weights in LDS, walking pixel tiles with the next tile's
 //                         input halo loaded while the current one is computed.
-//   conv3x3_wgrad_kernel  weight and bias gradients: per pixel chunk, 64-pixel tiles of the
-//                         output gradient and of the shifted input staged in LDS, FP32 outer
-//                         products per (tap, 4 x 4 channel block); partial sums per chunk
+//   conv3x3_wgrad_kernel  weight and bias gradients on MFMA: per (pixel chunk, tap), 64-pixel
+//                         tiles of the output gradient and of the shifted input staged in LDS
+//                         transposed to [channel][pixel]; grad_w = gout^T x_shift with K =
+//                         pixels, FP32 accumulation; partial sums per chunk
 //   conv3x3_wgrad_reduce  sums the chunk partials in a fixed order (deterministic)
 
 #include <hip/hip_runtime.h>
@@ -322,70 +323,97 @@ __global__ void __launch_bounds__(kPThreads) conv3x3_fwd_persistent_kernel(
     }
 }
 
-constexpr int kWgTile = 64;   // pixels per LDS tile
-constexpr int kWgPad = 65;
+constexpr int kWgTile = 64;     // pixels per LDS tile: two MFMA K steps of 32
+constexpr int kWgRow = 72;      // bf16 per LDS row (64 pixels + 8 of padding: conflict-free reads)
 
+// Weight and bias gradients of one tap on MFMA: grad_w[co][ci] = sum_p gout[p][co] x[p + tap][ci],
+// a 64 x 64 GEMM with K = pixels.  The workgroup of (pixel chunk, tap) stages 64-pixel tiles of
+// the output gradient and of the tap-shifted input in LDS transposed to [channel][pixel] (bf16,
+// rows padded to 72 so that the 16 rows a 16-lane group reads fall in distinct banks); wave w
+// accumulates rows 16 w .. 16 w + 15 of grad_w against all 64 columns with
+// v_mfma_f32_16x16x32_bf16 (lane l gives row/column l % 16 with pixels 8 (l / 16) .. +7 of the
+// K step and receives rows 4 (l / 16) .. +3 of column l % 16), FP32 accumulation.  Partial
+// sums per chunk are reduced in a fixed order (deterministic).
 __global__ void __launch_bounds__(256) conv3x3_wgrad_kernel(
     const bf16_t* __restrict__ in, const bf16_t* __restrict__ gout, float* __restrict__ part_w,
     float* __restrict__ part_b, int N, int H, int W, int Cin, int Cout, int chunk_px) {
-    __shared__ float sy[kWgTile][kWgPad];
-    __shared__ float sx[kWgTile][kWgPad];
+    __shared__ __attribute__((aligned(16))) bf16_t ty[kMaxCout * kWgRow];  // gout^T  [co][p]
+    __shared__ __attribute__((aligned(16))) bf16_t tx[kMaxCout * kWgRow];  // shifted x^T [ci][p]
     const int tap = blockIdx.y;
     const int chunk = blockIdx.x;
     const int dy = tap / 3 - 1, dx = tap % 3 - 1;
     const int P = N * H * W;
     const int tid = threadIdx.x;
-    const int co4 = (tid >> 4) * 4, ci4 = (tid & 15) * 4;
-    float acc[4][4];
+    const int lane = tid & 63, wave = tid >> 6;
+    const int m = lane & 15, g = lane >> 4;
+    f32x4 acc[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0f;
+    for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     float accb = 0.0f;
     const int p_begin = chunk * chunk_px;
     const int p_end = min(p_begin + chunk_px, P);
+    // staging: thread tid loads channels c8 .. c8+7 of the pixel pair (2 q, 2 q + 1) of the tile
+    const int q = tid >> 3, c8 = (tid & 7) * 8;
+    const bool vec_y = (Cout & 7) == 0, vec_x = (Cin & 7) == 0;
+    auto load8 = [&](const bf16_t* base, int C, int c0, bool vec) {
+        bf16x8 v = zero8();
+        if (vec) {
+            if (c0 < C) v = *reinterpret_cast<const bf16x8*>(base + c0);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+                if (c0 + e < C) v[e] = base[c0 + e];
+        }
+        return v;
+    };
     for (int p0 = p_begin; p0 < p_end; p0 += kWgTile) {
-        // stage: 64 pixels x 64 channels of the output gradient and of the shifted input
-        for (int e = tid; e < kWgTile * kMaxCout; e += 256) {
-            const int pl = e >> 6, c = e & 63;
-            const int p = p0 + pl;
-            float vy = 0.0f, vx = 0.0f;
+        bf16x8 vy[2], vx[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int p = p0 + 2 * q + h;
+            vy[h] = zero8();
+            vx[h] = zero8();
             if (p < p_end) {
-                if (c < Cout) vy = (float)gout[(size_t)p * Cout + c];
-                if (c < Cin) {
-                    const int x = p % W;
-                    const int t = p / W;
-                    const int y = t % H, n = t / H;
-                    const int yy = y + dy, xx = x + dx;
-                    if (yy >= 0 && yy < H && xx >= 0 && xx < W)
-                        vx = (float)in[((size_t)(n * H + yy) * W + xx) * Cin + c];
-                }
+                vy[h] = load8(gout + (size_t)p * Cout, Cout, c8, vec_y);
+                const int x = p % W;
+                const int t = p / W;
+                const int y = t % H, n = t / H;
+                const int yy = y + dy, xx = x + dx;
+                if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+                    vx[h] = load8(in + ((size_t)(n * H + yy) * W + xx) * Cin, Cin, c8, vec_x);
             }
-            sy[pl][c] = vy;
-            sx[pl][c] = vx;
+        }
+        // transposed store: channel c8 + e, pixels 2 q and 2 q + 1 in one 32-bit word
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int row = (c8 + e) * kWgRow + 2 * q;
+            bf16_t py2[2] = {vy[0][e], vy[1][e]};
+            bf16_t px2[2] = {vx[0][e], vx[1][e]};
+            *reinterpret_cast<uint32_t*>(&ty[row]) = *reinterpret_cast<const uint32_t*>(py2);
+            *reinterpret_cast<uint32_t*>(&tx[row]) = *reinterpret_cast<const uint32_t*>(px2);
         }
         __syncthreads();
-#pragma unroll 4
-        for (int pl = 0; pl < kWgTile; ++pl) {
-            float yv[4], xv[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                yv[i] = sy[pl][co4 + i];
-                xv[i] = sx[pl][ci4 + i];
+        for (int ks = 0; ks < kWgTile / 32; ++ks) {
+            const int pk = ks * 32 + 8 * g;
+            const bf16x8 a = *reinterpret_cast<const bf16x8*>(&ty[(16 * wave + m) * kWgRow + pk]);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const bf16x8 bb = *reinterpret_cast<const bf16x8*>(&tx[(16 * t + m) * kWgRow + pk]);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bb, acc[t], 0, 0, 0);
             }
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(yv[i], xv[j], acc[i][j]);
-            if (tap == 0 && tid < kMaxCout) accb += sy[pl][tid];
+        }
+        if (tap == 0 && tid < kMaxCout) {
+#pragma unroll 8
+            for (int p = 0; p < kWgTile; ++p) accb += (float)ty[tid * kWgRow + p];
         }
         __syncthreads();
     }
     float* pw = part_w + ((size_t)chunk * 9 + tap) * kMaxCout * kMaxCout;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int t = 0; t < 4; ++t)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) pw[(co4 + i) * kMaxCout + ci4 + j] = acc[i][j];
+        for (int r = 0; r < 4; ++r) pw[(16 * wave + 4 * g + r) * kMaxCout + 16 * t + m] = acc[t][r];
     if (tap == 0 && tid < kMaxCout) part_b[(size_t)chunk * kMaxCout + tid] = accb;
 }
 
