@@ -37,7 +37,7 @@ HBM_PEAK_GBS = 8000.0
 # the dominant kernel of one FDK step: the backprojection's common-case pass (the call also
 # launches a flag fill and the remainder pass, both under 1% of it for this workload; the HIP
 # events bracket all three, so the achieved rate is a lower bound for the pass itself)
-BP_KERNEL = "dd_backproject_tile_kernel<16, 0>"
+BP_KERNEL = "dd_backproject_tile_kernel<32, 0>"
 FP_KERNEL = "dd_forward_kernel"
 
 # The reference's own geometry file for its pixel-driven FDK (python/ReconstructionParams.txt:1-11,

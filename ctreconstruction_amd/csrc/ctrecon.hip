@@ -709,36 +709,39 @@ __global__ void bp_geometry_kernel(ctr_geometry g, const double* __restrict__ an
 // per (column, view) and reused along z; the reference's per-tap atomicAdd becomes a
 // register accumulation over all views and one store.
 //
-// Organisation: a workgroup owns a 16 x 16 x ZC voxel tile; for every view it stages the detector
-// window that the tile's bounding box projects to in LDS (double-buffered, one barrier per
-// view).  Each work-item computes the u-overlap weights of its voxel column once per view,
-// filters every window row it needs in u once, and keeps the running sum of those rows (a row
-// integral C(v), piecewise linear in the index-space coordinate v) in its own LDS column.  The
-// v-overlap-weighted sum of rows for a voxel whose footprint is [v_lo, v_hi] is then
+// Organisation: a workgroup owns a 16 x 16 x ZC voxel tile (ZC = 32); for every view it stages
+// the detector window that the tile's bounding box projects to in LDS (double-buffered, one
+// barrier per view).  Each work-item computes the u-footprint, u-overlap weights and FDK weight
+// of its voxel column once per view for all ZC voxels, which form two sub-chunks of 16.  Per
+// sub-chunk it filters every window row it needs in u once and keeps the running sum of those
+// rows (a row integral C(v), piecewise linear in the index-space coordinate v) in its own LDS
+// column.  The v-overlap-weighted sum of rows for a voxel whose footprint is [v_lo, v_hi] is then
 // (C(v_hi) - C(v_lo)) / (v_hi - v_lo): the reference's weights are exactly the overlap lengths
 // of the footprint with the rows, divided by its width.  Weight zeroing (numba variant) and
-// chunks spanning more than kMaxRows rows take the explicit per-row path instead.  Tiles are
-// assigned so that each XCD holds contiguous z-chunks, whose windows share detector rows in
+// sub-chunks spanning more than kMaxRows rows take the explicit per-row path instead.  Tiles
+// are assigned so that each XCD holds contiguous z-chunks, whose windows share detector rows in
 // that XCD's L2.  A window larger than the LDS buffer is gathered from global memory.
 constexpr int kTileX = 16, kTileY = 16;
 #ifndef CTR_BP_WIN_CAP
-#define CTR_BP_WIN_CAP 576
+#define CTR_BP_WIN_CAP 1024
 #endif
 constexpr int kWinCap = CTR_BP_WIN_CAP;  // floats per LDS window buffer
 #ifndef CTR_BP_WIN_RING
-#define CTR_BP_WIN_RING 128
+#define CTR_BP_WIN_RING 64
 #endif
 constexpr int kWinRing = CTR_BP_WIN_RING;  // views in the LDS ring of detector windows (power of 2)
 constexpr int kMaxUTaps = 3;           // u taps held in registers (more are looped)
 constexpr int kPrefetch = (kWinCap + 255) / 256;  // window cells per thread
 #ifndef CTR_BP_ZC
-#define CTR_BP_ZC 16
+#define CTR_BP_ZC 32
 #endif
 #ifndef CTR_BP_MAX_ROWS
-#define CTR_BP_MAX_ROWS 22
+#define CTR_BP_MAX_ROWS 19
 #endif
-constexpr int kBpZC = CTR_BP_ZC;            // voxels along z per work-item
-constexpr int kMaxRows = CTR_BP_MAX_ROWS;   // rows per column chunk on the row-integral path
+constexpr int kBpZC = CTR_BP_ZC;            // voxels along z per work-item (and tile)
+constexpr int kBpSub = 16;                  // voxels per sub-chunk: the unit of one row integral
+static_assert(kBpZC % kBpSub == 0, "a work-item's voxels split into whole sub-chunks");
+constexpr int kMaxRows = CTR_BP_MAX_ROWS;   // rows per sub-chunk on the row-integral path
 
 struct BpView {
     float c, s, Sx, Sy, Sz, Au, Bu, Av, Bv, R;
@@ -870,21 +873,33 @@ __device__ inline BpChunk bp_chunk(const BpView& v, const BpColumn& col, float Z
     return ch;
 }
 
-// Row integral C at row boundaries ks .. ks+nr of the window rows from `rp` (stride `stride`),
-// u-filtered with three taps (weight 0 past the footprint; the window buffers are zero-
-// initialised and padded, so such reads stay finite) and times W, in this work-item's LDS
-// column `scan` (stride 256), plus a copy of the total at ks+nr+1 so that a lookup at the top
-// needs no index clamp.  Two rows per step: both rows' loads go out before one wait.
+// Row-integral table of a work-item (its LDS column `scan`, stride 256 floats): entry m holds
+// C at row boundary ks - 1 + m, so entries 0 and 1 are 0 (boundaries ks - 1 and ks), entry j + 2
+// the integral of rows ks .. ks + j, and entry nr + 2 repeats the total.  A lookup at
+// x = v - (ks - 1) interpolates entries floor(x) and floor(x) + 1: for x in [0, nr + 2) it needs
+// no clamp, which leaves a row of slack on either side of the rows [ks, ks + nr] for rounding.
+__device__ inline float bp_lookup(const float* __restrict__ scan, float x) {
+    const int jj = (int)x;
+    const float a = scan[jj * 256], b = scan[(jj + 1) * 256];
+    return fmaf(__builtin_amdgcn_fractf(x), b - a, a);
+}
+
+// The row integral of nr window rows from `rp` (stride `stride`), u-filtered with three taps
+// (weight 0 past the footprint; the window buffers are zero-initialised and padded, so such
+// reads stay finite) and times W, into the table above.  Two rows per step.
 __device__ inline void bp_row_integral3(const float* __restrict__ rp, int stride, int nr, const float (&wu)[kMaxUTaps],
                                         float W, float* __restrict__ scan) {
     float csum = 0.0f;
     scan[0] = 0.0f;
-    float* sp = scan;
+    scan[256] = 0.0f;
+    float* sp = scan + 256;
     int j = 0;
     for (; j + 1 < nr; j += 2) {
         const float* rq = rp + stride;
-        const float rk0 = fmaf(wu[0], rp[0], fmaf(wu[1], rp[1], wu[2] * rp[2]));
-        const float rk1 = fmaf(wu[0], rq[0], fmaf(wu[1], rq[1], wu[2] * rq[2]));
+        const float p0 = rp[0], p1 = rp[1], p2 = rp[2];
+        const float q0 = rq[0], q1 = rq[1], q2 = rq[2];
+        const float rk0 = fmaf(wu[0], p0, fmaf(wu[1], p1, wu[2] * p2));
+        const float rk1 = fmaf(wu[0], q0, fmaf(wu[1], q1, wu[2] * q2));
         csum = fmaf(rk0, W, csum);
         sp[256] = csum;
         csum = fmaf(rk1, W, csum);
@@ -897,40 +912,70 @@ __device__ inline void bp_row_integral3(const float* __restrict__ rp, int stride
         csum = fmaf(rk, W, csum);
         sp[256] = csum;
     }
-    scan[(nr + 1) * 256] = csum;
+    scan[(nr + 2) * 256] = csum;
 }
 
-// The voxels of a chunk with affine footprints from the row integral: C between row boundaries
-// is linear, and clamping v - ks to [0, nr] gives C = 0 below and the total above (rows outside
-// the detector or the window contribute nothing).  1/(hi - lo) of voxel i comes by one Newton
-// step from voxel i-1's: the footprint height is affine in i and changes by a small fraction d
-// per voxel (the magnification spread over one voxel), so the step leaves a relative error of
-// about d^2 (below 1e-8 at 512^3) and costs two FMAs instead of a reciprocal, which issues at
-// half the FMA rate.
-template <int ZC>
+// The voxels of a sub-chunk (acc[OFF .. OFF+kBpSub-1]) with affine footprints from the row
+// integral: C between row boundaries is linear.  With CLAMP, v - (ks - 1) is clamped to
+// [1, nr + 1], which gives C = 0 below and the total above (rows outside the detector or the
+// window contribute nothing).  Without it the footprints of all kBpSub voxels must lie inside
+// the table's rows (bp_sub_rows' `inside`: the sub-chunk is whole and its rows were not cut by
+// the detector or the window); the clamp then never binds, so both ways give the same sums.
+// 1/(hi - lo) of voxel i comes by one Newton step from voxel i-1's: the footprint height is
+// affine in i and changes by a small fraction d per voxel (the magnification spread over one
+// voxel), so the step leaves a relative error of about d^2 (below 1e-8 at 512^3) and costs two
+// FMAs instead of a reciprocal, which issues at half the FMA rate.
+template <int OFF, bool CLAMP, int ZC>
 __device__ inline void bp_voxels_affine(const BpChunk& ch, int ks, int nr, const float* __restrict__ scan,
                                         float (&acc)[ZC]) {
-    const float fks = (float)ks, fnr = (float)nr;
-    auto integral = [&](float x) {
-        const float f = __builtin_amdgcn_fmed3f(x, 0.0f, fnr);
-        const int jj = (int)f;
-        const float a = scan[jj * 256], b = scan[(jj + 1) * 256];
-        return fmaf(__builtin_amdgcn_fractf(f), b - a, a);
-    };
-    const float alo = ch.Alo - fks, ahi = ch.Ahi - fks;
+    static_assert(OFF + kBpSub <= ZC, "sub-chunk inside the work-item");
+    const float base = (float)(ks - 1), top = (float)(nr + 1);
+    const float alo = ch.Alo - base, ahi = ch.Ahi - base;
     float rinv = __builtin_amdgcn_rcpf(ahi - alo);
 #pragma unroll
-    for (int i = 0; i < ZC; ++i) {
+    for (int i = 0; i < kBpSub; ++i) {
         const float lo = fmaf((float)i, ch.Dlo, alo), hi = fmaf((float)i, ch.Dhi, ahi);
         if (i > 0) rinv *= fmaf(lo - hi, rinv, 2.0f);
-        acc[i] = fmaf(integral(hi) - integral(lo), rinv, acc[i]);
+        const float xl = CLAMP ? __builtin_amdgcn_fmed3f(lo, 1.0f, top) : lo;
+        const float xh = CLAMP ? __builtin_amdgcn_fmed3f(hi, 1.0f, top) : hi;
+        acc[OFF + i] = fmaf(bp_lookup(scan, xh) - bp_lookup(scan, xl), rinv, acc[OFF + i]);
     }
 }
 
-// The fast pass (kBpFast): the common case only -- window in LDS, chunk on one side of the
+// z of the first voxel of sub-chunk h of a work-item whose first voxel is at Z0: every pass
+// derives its sub-chunks from this, so they round them alike
+__device__ inline float bp_sub_z0(float Z0, float dz, int h) { return fmaf((float)(kBpSub * h), dz, Z0); }
+
+// Rows [ks, ks + nr - 1] (clamped to the detector and to the rows [row0, row0 + nrows) of the
+// source) that the n voxels of an affine sub-chunk touch: the ends move monotonically with i,
+// so the range comes from the end voxels.  `inside`: the sub-chunk is whole and nothing was cut.
+__device__ inline void bp_sub_rows(const BpChunk& ch, int n, int row0, int nrows, int nv, int& ks, int& nr,
+                                   bool& inside) {
+    const float fL = (float)(n - 1);
+    const float lo0 = fmaf(0.0f, ch.Dlo, ch.Alo), hi0 = fmaf(0.0f, ch.Dhi, ch.Ahi);
+    const float loL = fmaf(fL, ch.Dlo, ch.Alo), hiL = fmaf(fL, ch.Dhi, ch.Ahi);
+    const int klo = (int)floorf(fminf(lo0, loL)), khi = (int)floorf(fmaxf(hi0, hiL));
+    ks = max(max(klo, 0), row0);
+    const int ke = min(min(khi, nv - 1), row0 + nrows - 1);
+    nr = ke - ks + 1;
+    inside = n == kBpSub && ks == klo && ke == khi;
+}
+
+// Calls f(integral_constant<int, h>) for every sub-chunk h of a ZC-voxel work-item.
+template <int ZC, int H = 0, typename F>
+__device__ inline void bp_for_subs(F&& f) {
+    if constexpr (H * kBpSub < ZC) {
+        f(std::integral_constant<int, H>{});
+        bp_for_subs<ZC, H + 1>(f);
+    }
+}
+
+// The fast pass (kBpFast): the common case only -- window in LDS, sub-chunk on one side of the
 // source height, at most kMaxRows rows and kMaxUTaps columns, no weight zeroing (the host runs
-// this pass only with zero_tol = 0).  Returns true when it leaves the (work-item, view) to the
-// completing pass, which decides from the same bp_column / bp_chunk values.
+// this pass only with zero_tol = 0).  The column's u-footprint, u-weights and FDK weight are
+// worked out once per view for all ZC voxels; every sub-chunk then builds its own row integral.
+// Returns true when it leaves a sub-chunk of this (work-item, view) to the completing pass,
+// which decides from the same bp_column / bp_chunk values.
 template <int ZC>
 __device__ inline bool bp_column_view_fast(const BpView& v, const float* __restrict__ win, int stride,
                                            int row0, int nrows, int col0, int ncols, float X, float Y,
@@ -941,18 +986,7 @@ __device__ inline bool bp_column_view_fast(const BpView& v, const float* __restr
     // taps inside the detector and the window
     const int kus = max(max(col.ku0, 0), col0), kue = min(min(col.ku1, nu - 1), col0 + ncols - 1);
     if (kus > kue) return false;
-    const BpChunk ch = bp_chunk<ZC>(v, col, Z0, dz, hz);
-    if (!ch.lin) return true;
-    // the ends move monotonically with i, so the chunk's row range comes from its end voxels
-    const float fL = (float)(nz_left - 1);
-    const float lo0 = fmaf(0.0f, ch.Dlo, ch.Alo), hi0 = fmaf(0.0f, ch.Dhi, ch.Ahi);
-    const float loL = fmaf(fL, ch.Dlo, ch.Alo), hiL = fmaf(fL, ch.Dhi, ch.Ahi);
-    const float rlo = fminf(lo0, loL), rhi = fmaxf(hi0, hiL);
-    const int ks = max(max((int)floorf(rlo), 0), row0);
-    const int ke = min(min((int)floorf(rhi), nv - 1), row0 + nrows - 1);
-    const int nr = ke - ks + 1;
-    if (nr <= 0) return false;
-    if (nr > kMaxRows || kue - kus + 1 > kMaxUTaps) return true;
+    const bool taps_ok = kue - kus + 1 <= kMaxUTaps;
     // overlap of each tap with [ulo, uhi] over the footprint width (python/GPUFuncs_pycuda.py:
     // 427-438 without zeroing); a footprint inside one cell weighs 1
     const float inv_u = __builtin_amdgcn_rcpf(col.uhi - col.ulo);
@@ -967,14 +1001,34 @@ __device__ inline bool bp_column_view_fast(const BpView& v, const float* __restr
     if (col.ku0 == col.ku1) wu[0] = 1.0f;
     const float dd = v.R - col.yr0;
     const float W = scale * (v.R * v.R) * __builtin_amdgcn_rcpf(dd * dd);
-    // window offsets fit 24 bits: a full-rate multiply (the 32-bit one is quarter rate)
-    bp_row_integral3(win + __mul24(ks - row0, stride) + (kus - col0), stride, nr, wu, W, scan);
-    bp_voxels_affine<ZC>(ch, ks, nr, scan, acc);
-    return false;
+    const float* wp = win + (kus - col0);
+    bool left = false;
+    bp_for_subs<ZC>([&](auto hc) {
+        constexpr int h = decltype(hc)::value;
+        const int n = min(kBpSub, nz_left - kBpSub * h);
+        if (n <= 0) return;
+        const BpChunk ch = bp_chunk<kBpSub>(v, col, bp_sub_z0(Z0, dz, h), dz, hz);
+        if (!ch.lin) {
+            left = true;
+            return;
+        }
+        int ks, nr;
+        bool inside;
+        bp_sub_rows(ch, n, row0, nrows, nv, ks, nr, inside);
+        if (nr <= 0) return;
+        if (nr > kMaxRows || !taps_ok || !inside) {
+            left = true;
+            return;
+        }
+        // window offsets fit 24 bits: a full-rate multiply (the 32-bit one is quarter rate)
+        bp_row_integral3(wp + __mul24(ks - row0, stride), stride, nr, wu, W, scan);
+        bp_voxels_affine<kBpSub * h, false>(ch, ks, nr, scan, acc);
+    });
+    return left;
 }
 
-// The completing pass (kBpRest: every (work-item, view) the fast pass left, and nothing else)
-// and the single pass over all cases (kBpAll).
+// The completing pass (kBpRest: every (work-item, view, sub-chunk) the fast pass left, and
+// nothing else) and the single pass over all cases (kBpAll).
 template <int ZC, bool FROM_LDS, int KIND>
 __device__ inline void bp_column_view(const BpView& v, const float* __restrict__ src, int stride,
                                       int row0, int nrows, int col0, int ncols, float X, float Y,
@@ -997,99 +1051,112 @@ __device__ inline void bp_column_view(const BpView& v, const float* __restrict__
     }
     const float dd = v.R - col.yr0;
     const float W = scale * (v.R * v.R) * __builtin_amdgcn_rcpf(dd * dd);
-    const BpChunk ch = bp_chunk<ZC>(v, col, Z0, dz, hz);
-    const bool lin = ch.lin;
     const float rmin = col.rmin, rmax = col.rmax;
-    auto foot = [&](float fi, float& lo, float& hi) {
-        if (lin) {
-            lo = fmaf(fi, ch.Dlo, ch.Alo);
-            hi = fmaf(fi, ch.Dhi, ch.Ahi);
-        } else {
-            const float e = v.Sz - fmaf(fi, dz, Z0);
-            const float ep = e + hz, em = e - hz;
-            const float smx = ep * (ep >= 0.0f ? rmax : rmin);
-            const float smn = em * (em >= 0.0f ? rmin : rmax);
-            const float g1 = fmaf(-v.Bv, smn, v.Av), g2 = fmaf(-v.Bv, smx, v.Av);
-            lo = fminf(g1, g2);
-            hi = fmaxf(g1, g2);
-        }
-    };
-    // the ends move monotonically with i, so the chunk's row range comes from its end voxels
-    float lo0, hi0, loL, hiL;
-    foot(0.0f, lo0, hi0);
-    foot((float)(nz_left - 1), loL, hiL);
-    const float rlo = fminf(lo0, loL), rhi = fmaxf(hi0, hiL);
-    const int ks = max(max((int)floorf(rlo), 0), row0);
-    const int ke = min(min((int)floorf(rhi), nv - 1), row0 + nrows - 1);
-    const int nr = ke - ks + 1;
-    if (nr <= 0) return;
     const int cbase = kus - col0;
     const int ntaps = kue - kus + 1;
-    if constexpr (KIND == kBpRest) {
-        // the pairs the fast pass did
-        const bool common = window_in_lds && lin && nr <= kMaxRows && ntaps <= kMaxUTaps && zero_tol <= 0.0f;
-        if (common) return;
-    }
-    if (zero_tol <= 0.0f && nr <= kMaxRows) {
-        const float* rp = src + (FROM_LDS ? __mul24(ks - row0, stride) : (size_t)(ks - row0) * stride) + cbase;
-        if (FROM_LDS && !__any(ntaps > kMaxUTaps)) {
-            bp_row_integral3(rp, stride, nr, wu, W, scan);
-        } else {
-            float csum = 0.0f;
-            scan[0] = 0.0f;
-            for (int j = 0; j < nr; ++j) {
-                float rk = 0.0f;
-#pragma unroll
-                for (int t = 0; t < kMaxUTaps; ++t) rk = fmaf(wu[t], rp[min(t, ntaps - 1)], rk);
-                for (int t = kMaxUTaps; t < ntaps; ++t)
-                    rk = fmaf(dd_weight(kus + t, ku0, ku1, ulo, uhi, inv_u, zero_tol), rp[t], rk);
-                csum = fmaf(rk, W, csum);
-                scan[(j + 1) * 256] = csum;
-                rp += stride;
+    bp_for_subs<ZC>([&](auto hc) {
+        constexpr int h = decltype(hc)::value;
+        constexpr int OFF = kBpSub * h;
+        const int n = min(kBpSub, nz_left - kBpSub * h);
+        if (n <= 0) return;
+        const float Z0h = bp_sub_z0(Z0, dz, h);
+        const BpChunk ch = bp_chunk<kBpSub>(v, col, Z0h, dz, hz);
+        const bool lin = ch.lin;
+        auto foot = [&](float fi, float& lo, float& hi) {
+            if (lin) {
+                lo = fmaf(fi, ch.Dlo, ch.Alo);
+                hi = fmaf(fi, ch.Dhi, ch.Ahi);
+            } else {
+                const float e = v.Sz - fmaf(fi, dz, Z0h);
+                const float ep = e + hz, em = e - hz;
+                const float smx = ep * (ep >= 0.0f ? rmax : rmin);
+                const float smn = em * (em >= 0.0f ? rmin : rmax);
+                const float g1 = fmaf(-v.Bv, smn, v.Av), g2 = fmaf(-v.Bv, smx, v.Av);
+                lo = fminf(g1, g2);
+                hi = fmaxf(g1, g2);
             }
-            scan[(nr + 1) * 256] = csum;
-        }
+        };
+        int ks, nr;
+        bool inside = false;
         if (lin) {
-            bp_voxels_affine<ZC>(ch, ks, nr, scan, acc);
+            bp_sub_rows(ch, n, row0, nrows, nv, ks, nr, inside);
         } else {
-            const float fks = (float)ks, fnr = (float)nr;
-            auto integral = [&](float x) {
-                const float f = __builtin_amdgcn_fmed3f(x, 0.0f, fnr);
-                const int jj = (int)f;
-                const float a = scan[jj * 256], b = scan[(jj + 1) * 256];
-                return fmaf(__builtin_amdgcn_fractf(f), b - a, a);
-            };
+            // the ends move monotonically with i, so the range comes from the end voxels
+            float lo0, hi0, loL, hiL;
+            foot(0.0f, lo0, hi0);
+            foot((float)(n - 1), loL, hiL);
+            const float rlo = fminf(lo0, loL), rhi = fmaxf(hi0, hiL);
+            ks = max(max((int)floorf(rlo), 0), row0);
+            const int ke = min(min((int)floorf(rhi), nv - 1), row0 + nrows - 1);
+            nr = ke - ks + 1;
+        }
+        if (nr <= 0) return;
+        if constexpr (KIND == kBpRest) {
+            // the sub-chunks the fast pass did
+            const bool common =
+                window_in_lds && lin && inside && nr <= kMaxRows && ntaps <= kMaxUTaps && zero_tol <= 0.0f;
+            if (common) return;
+        }
+        if (zero_tol <= 0.0f && nr <= kMaxRows) {
+            const float* rp = src + (FROM_LDS ? __mul24(ks - row0, stride) : (size_t)(ks - row0) * stride) + cbase;
+            if (FROM_LDS && !__any(ntaps > kMaxUTaps)) {
+                bp_row_integral3(rp, stride, nr, wu, W, scan);
+            } else {
+                float csum = 0.0f;
+                scan[0] = 0.0f;
+                scan[256] = 0.0f;
+                for (int j = 0; j < nr; ++j) {
+                    float rk = 0.0f;
 #pragma unroll
-            for (int i = 0; i < ZC; ++i) {
-                float lo, hi;
-                foot((float)i, lo, hi);
-                lo -= fks;
-                hi -= fks;
-                acc[i] = fmaf(integral(hi) - integral(lo), __builtin_amdgcn_rcpf(hi - lo), acc[i]);
+                    for (int t = 0; t < kMaxUTaps; ++t) rk = fmaf(wu[t], rp[min(t, ntaps - 1)], rk);
+                    for (int t = kMaxUTaps; t < ntaps; ++t)
+                        rk = fmaf(dd_weight(kus + t, ku0, ku1, ulo, uhi, inv_u, zero_tol), rp[t], rk);
+                    csum = fmaf(rk, W, csum);
+                    scan[(j + 2) * 256] = csum;
+                    rp += stride;
+                }
+                scan[(nr + 2) * 256] = csum;
+            }
+            if (lin) {
+                bp_voxels_affine<OFF, true>(ch, ks, nr, scan, acc);
+            } else {
+                const float base = (float)(ks - 1), top = (float)(nr + 1);
+#pragma unroll
+                for (int i = 0; i < kBpSub; ++i) {
+                    float lo, hi;
+                    foot((float)i, lo, hi);
+                    lo -= base;
+                    hi -= base;
+                    const float xl = __builtin_amdgcn_fmed3f(lo, 1.0f, top), xh = __builtin_amdgcn_fmed3f(hi, 1.0f, top);
+                    acc[OFF + i] = fmaf(bp_lookup(scan, xh) - bp_lookup(scan, xl), __builtin_amdgcn_rcpf(hi - lo),
+                                        acc[OFF + i]);
+                }
+            }
+            return;
+        }
+        // explicit per-row weights (zeroing of small weights, or a sub-chunk over more than
+        // kMaxRows rows): rare, so the voxel footprints are recomputed per row rather than held
+        // in registers
+        const int ke = ks + nr - 1;
+        for (int k = ks; k <= ke; ++k) {
+            const float* row = src + (size_t)(k - row0) * stride + cbase;
+            float rk = 0.0f;
+#pragma unroll
+            for (int j = 0; j < kMaxUTaps; ++j) rk = fmaf(wu[j], row[min(j, ntaps - 1)], rk);
+            for (int j = kMaxUTaps; j < ntaps; ++j)
+                rk = fmaf(dd_weight(kus + j, ku0, ku1, ulo, uhi, inv_u, zero_tol), row[j], rk);
+            rk *= W;
+            const float fk = (float)k, fk1 = (float)(k + 1);
+#pragma unroll
+            for (int i = 0; i < kBpSub; ++i) {
+                float vlo, vhi;
+                foot((float)i, vlo, vhi);
+                float w = (fminf(vhi, fk1) - fmaxf(vlo, fk)) * __builtin_amdgcn_rcpf(vhi - vlo);
+                w = (w < zero_tol || w < 0.0f) ? 0.0f : w;
+                acc[OFF + i] = fmaf(w, rk, acc[OFF + i]);
             }
         }
-        return;
-    }
-    // explicit per-row weights (zeroing of small weights, or a chunk over more than kMaxRows
-    // rows): rare, so the voxel footprints are recomputed per row rather than held in registers
-    for (int k = ks; k <= ke; ++k) {
-        const float* row = src + (size_t)(k - row0) * stride + cbase;
-        float rk = 0.0f;
-#pragma unroll
-        for (int j = 0; j < kMaxUTaps; ++j) rk = fmaf(wu[j], row[min(j, ntaps - 1)], rk);
-        for (int j = kMaxUTaps; j < ntaps; ++j)
-            rk = fmaf(dd_weight(kus + j, ku0, ku1, ulo, uhi, inv_u, zero_tol), row[j], rk);
-        rk *= W;
-        const float fk = (float)k, fk1 = (float)(k + 1);
-#pragma unroll
-        for (int i = 0; i < ZC; ++i) {
-            float vlo, vhi;
-            foot((float)i, vlo, vhi);
-            float w = (fminf(vhi, fk1) - fmaxf(vlo, fk)) * __builtin_amdgcn_rcpf(vhi - vlo);
-            w = (w < zero_tol || w < 0.0f) ? 0.0f : w;
-            acc[i] = fmaf(w, rk, acc[i]);
-        }
-    }
+    });
 }
 
 // KIND: kBpFast writes the volume and sets flags[tile][view] for every view it leaves (in
@@ -1102,7 +1169,7 @@ __global__ void __launch_bounds__(256) dd_backproject_tile_kernel(
     int accumulate, int tiles_x, int tiles_y, int tiles_z) {
     __shared__ float win[2][kWinCap + kMaxUTaps - 1];
     __shared__ int4 wring[kWinRing];
-    __shared__ float scan_lds[(kMaxRows + 2) * 256];
+    __shared__ float scan_lds[(kMaxRows + 3) * 256];
     float* scan = scan_lds + threadIdx.x;
     // XCD-aware tile order: workgroup b runs on XCD b % 8; give each XCD a contiguous range
     const int T = tiles_x * tiles_y * tiles_z;
@@ -1144,9 +1211,10 @@ __global__ void __launch_bounds__(256) dd_backproject_tile_kernel(
 
     // Views are taken in pairs: the windows of views v+2 and v+3 are loaded into registers while
     // views v and v+1 are computed from the group's two LDS windows, then stored into them between
-    // two barriers, so a window load has two views of work to arrive.  Two windows rather than
-    // four, a 128-view ring and 22 scratch rows keep the group at 31 KB of LDS, so five groups
-    // share a CU instead of four (73.8 ms instead of 83.1 at 512^3 x 720 views).
+    // two barriers, so a window load has two views of work to arrive.  Two windows of 1024 cells,
+    // a 64-view ring and 22 table rows per work-item keep the group at 31 KB of LDS, so five
+    // groups share a CU: one more table row (32 KB) leaves room for four, 7 % slower at
+    // 512^3 x 720 views.
     // The windows depend on the tile and the view only: they are worked out cooperatively, one
     // view per thread, kWinRing/2 views ahead, into a ring in LDS that every thread reads.
     auto fill_windows = [&](int first, int count) {
@@ -1287,7 +1355,7 @@ __global__ void __launch_bounds__(256) fill_zero_kernel(uint32_t* __restrict__ p
 // Matched adjoint of the forward projector
 // ------------------------------------------------------------------------------------------
 
-// The exact transpose of dd_forward_column_kernel, as a gather over voxels (no atomics).  No reference
+// The exact transpose of dd_forward_kernel, as a gather over voxels (no atomics).  No reference
 // counterpart: the reference backprojects with the FDK-weighted operator, which is not A^T.
 // For one view a voxel sits in slab n at transverse cell ka and z cell kc (index space of the
 // forward projector's tables).  The detector columns whose U-edge lines bracket [ka, ka+1] at

@@ -355,17 +355,18 @@ def test_numba_variant_planes_on_phantom_centre(dev):
     assert _rel(R.proj, O.forward_project(g_bp, vol.astype(np.float64))) > 10 * TOL
 
 
-# Volumes of three 16-voxel z chunks whose (tile, view) pairs fall partly outside the
-# backprojector's common case, so that the completing pass has work:
+# Volumes of 48 slices (a 16 x 16 x 32 voxel tile of two 16-voxel sub-chunks over a tile of one)
+# whose (tile, view) pairs fall partly outside the backprojector's common case, so that the
+# completing pass has work:
 SPLIT_CASES = {
-    # a chunk spans 16 * 0.5 * mag / 0.55 = 21.6 .. 22.1 rows (+1 for its partial end rows):
-    # some chunks fit the 22-row limit of the row-integral scratch, some do not
-    "rows_over_22": dict(NumberOfImage=[32, 32, 48], NumberOfDetectorPixels=[40, 80],
-                         DetectorPixelSize=[1.0, 0.55], NumberOfViews=10, StartAngle=0.05,
+    # a sub-chunk spans 16 * 0.5 * mag / 0.63 = 18.8 .. 19.3 rows (+1 for its partial end rows):
+    # some sub-chunks fit the 19-row limit of the row-integral scratch, some do not
+    "rows_over_19": dict(NumberOfImage=[32, 32, 48], NumberOfDetectorPixels=[40, 80],
+                         DetectorPixelSize=[1.0, 0.63], NumberOfViews=10, StartAngle=0.05,
                          EndAngle=0.05 + 2 * np.pi),
     # a voxel covers 0.5 * mag / 0.3 = 2.5 (axis-aligned) .. 3.5 (diagonal) cells: more than 3
-    # u taps at some views; a 16 x 16 tile's window is 40 .. 57 cells (+2 of margin) by about
-    # 9 rows (+2), so some windows fit the 576-cell LDS buffer and some do not
+    # u taps at some views; a 16 x 16 x 32 tile's window is 40 .. 57 cells (+2 of margin) by
+    # about 17 rows (+2), so some windows fit the 1024-cell LDS buffer and some do not
     "u_taps_and_windows": dict(NumberOfImage=[32, 32, 48], NumberOfDetectorPixels=[120, 32],
                                DetectorPixelSize=[0.3, 1.4], NumberOfViews=10, StartAngle=0.05,
                                EndAngle=0.05 + 2 * np.pi),
@@ -374,14 +375,14 @@ SPLIT_CASES = {
 
 def test_split_cases_reach_the_completing_pass():
     mag_lo, mag_hi = 1500.0 / (1000.0 + 8 * np.sqrt(2)), 1500.0 / (1000.0 - 8 * np.sqrt(2))
-    p = SPLIT_CASES["rows_over_22"]
+    p = SPLIT_CASES["rows_over_19"]
     rows = 16 * 0.5 / p["DetectorPixelSize"][1]
-    assert rows * mag_lo < 22 < rows * mag_hi + 1
+    assert rows * mag_lo < 19 < rows * mag_hi + 1
     p = SPLIT_CASES["u_taps_and_windows"]
     du = p["DetectorPixelSize"][0]
     assert 0.5 * mag_lo / du < 3 < 0.5 * np.sqrt(2) * mag_hi / du
-    rows = 16 * 0.5 * mag_hi / p["DetectorPixelSize"][1] + 2
-    assert (8 * mag_lo / du + 2) * rows < 576 < (8 * np.sqrt(2) * mag_hi / du + 2) * rows
+    rows = 32 * 0.5 * mag_hi / p["DetectorPixelSize"][1] + 2
+    assert (8 * mag_lo / du + 2) * rows < 1024 < (8 * np.sqrt(2) * mag_hi / du + 2) * rows
 
 
 @pytest.mark.parametrize("name", sorted(SPLIT_CASES) + ["flat"])

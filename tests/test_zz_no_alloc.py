@@ -66,10 +66,10 @@ def test_static_scratch_matches_workspace_entry_point(dev, mode):
 
 def test_static_scratch_rejects_what_it_cannot_hold(dev):
     lib = N.lib()
-    # 512^3 in 16^3 tiles is 32768 tiles: 1100 views need 36 MB of flags
+    # 512^3 in 16 x 16 x 32 tiles is 16384 tiles: 2200 views need 36 MB of flags
     geom = ctr.ScanGeometry(make_params(NumberOfImage=[512, 512, 512], ImagePixelSpacing=[0.25] * 3,
                                         NumberOfDetectorPixels=[1024, 768], DetectorPixelSize=[0.4, 0.4],
-                                        NumberOfViews=1100))
+                                        NumberOfViews=2200))
     g = geom.ctypes_struct()
     assert lib.ctr_dd_backproject_workspace_bytes(ctypes.byref(g), 0) > 32 << 20
     null = ctypes.c_void_p(0)
