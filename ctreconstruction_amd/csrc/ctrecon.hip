@@ -3,7 +3,7 @@
 // Kernels (all FP32 arithmetic; FP64 only in the per-view geometry kernels):
 //   fp_geometry_kernel          per-view DD footprint tables, FP64 -> FP32 (index space)
 //   zprefix_kernel              vol[z][y][x] -> z-prefix tables [y][x][k] and [x][y][k]
-//   dd_forward_kernel           one wavefront per (view, 4 detector columns, 63 rows)
+//   dd_forward_kernel           one wavefront per (2 views, 4 detector columns, 63 rows)
 //   bp_geometry_kernel          per-view backprojection constants, FP64 -> FP32
 //   dd_backproject_tile_kernel  one workgroup per 16 x 16 x ZC voxel tile, windows in LDS
 //   adj_geometry_kernel         per-view constants of the inverse central projection
@@ -244,31 +244,36 @@ __global__ void __launch_bounds__(256) zprefix_kernel(const float* __restrict__ 
 // are in index space (plane k at k), so the same code serves increasing (x) and decreasing
 // (y, z) planes.
 //
-// One wavefront per (view, kFpCols adjacent detector columns, kColRows rows).  All rays of a
-// column share their transverse footprint at every slab (the U-edge lines depend on the column
-// only), so the transverse overlaps and the lines they cover are uniform across the wavefront.
-// In a batch of kFpBatch slabs, lane 4 s + c works them out for slab n0 + s and column c, the
-// four lanes of slab s agree (quad swizzles) on the union of their columns' lines and z
-// breakpoints, and the wavefront then walks the batch slab by slab.
+// One wavefront per (two adjacent views, kFpCols adjacent detector columns, kColRows rows).
+// All rays of a column share their transverse footprint at every slab (the U-edge lines depend
+// on the column only), so the transverse overlaps and the lines they cover are uniform across
+// the wavefront.  In a batch of kFpBatch slabs, lane 4 s + c works them out for slab n0 + s,
+// column c and both views; the four lanes of slab s agree (quad swizzles) on the union of the
+// eight column-views' lines and z breakpoints, and the wavefront then walks the batch slab by
+// slab.  Adjacent views cross nearly the same lines (their rays turn by one angular step, which
+// moves a ray's crossing with a slab by a cell or two), so one staged segment serves both: the
+// prefix data fetched from beyond L2 per view falls by about a third.  The two views of a pair
+// are projected together when they use the same slab orientation; a pair that straddles a
+// change of orientation walks the slabs once per view.
 //
 // z direction: the footprint of row k at a slab is [c_k, c_k+1] with c_e = nu_e tt + gamma on
 // the row edges e, and its overlap-weighted sum over a z line is the line's z integral
 // I(c_k+1) - I(c_k), interpolated in the z-prefix table (the two breakpoints around the edge).
-// Per slab the wavefront stages the z segment of the union of its columns' lines in LDS with
-// one to three coalesced 16-byte-per-lane loads (lanes past the segment get an out-of-range
-// offset and fetch nothing): adjacent columns share most of their lines, so one load serves
-// about two columns.  Lane L owns edge e0 + L of kColRows + 1 = 64 consecutive edges and sums,
-// in FP64 over the slabs and per column, H_e = sum_slab sum_a w_a I_a(c_e) with the transverse
-// weights w_a pre-divided by wa * tt; row k is then (H_k+1 - H_k) / (nu_k+1 - nu_k), scaled by
-// the ray's length factor, and the four columns of a row leave in one 16-byte store.  The
-// width |c_k+1 - c_k| = |nu_k+1 - nu_k| |tt| is the reference's normaliser; a footprint inside
-// one cell gets weight 1 (up to rounding) as in the reference, and cells outside the volume
-// are cut by clamping c_e to [0, nz].  Transverse overlaps below zero_tol times the footprint
-// width are dropped as in the reference; z overlaps are not split per cell, so the reference's
-// per-cell zeroing of z weights below zero_tol (1e-6 of the footprint) has no counterpart.
-// Footprints over three cells, and unions too large for the LDS segment, take an explicit
-// per-line path that reads the breakpoint pairs straight from memory.
+// Per slab the wavefront stages the z segment of the union of its column-views' lines in LDS
+// with one to four coalesced 16-byte-per-lane loads (lanes past the segment get an out-of-range
+// offset and fetch nothing).  Lane L owns edge e0 + L of kColRows + 1 = 64 consecutive edges and
+// sums, in FP64 over the slabs and per column-view, H_e = sum_slab sum_a w_a I_a(c_e) with the
+// transverse weights w_a pre-divided by wa * tt; row k is then (H_k+1 - H_k) / (nu_k+1 - nu_k),
+// scaled by the ray's length factor, and the four columns of a row leave in one 16-byte store.
+// The width |c_k+1 - c_k| = |nu_k+1 - nu_k| |tt| is the reference's normaliser; a footprint
+// inside one cell gets weight 1 (up to rounding) as in the reference, and cells outside the
+// volume are cut by clamping c_e to [0, nz].  Transverse overlaps below zero_tol times the
+// footprint width are dropped as in the reference; z overlaps are not split per cell, so the
+// reference's per-cell zeroing of z weights below zero_tol (1e-6 of the footprint) has no
+// counterpart.  Footprints over three cells, and unions too large for the LDS segment, take an
+// explicit per-line path that reads the breakpoint pairs straight from memory.
 constexpr int kFpCols = 4;              // detector columns per wavefront
+constexpr int kFpViews = 2;             // adjacent views per wavefront
 constexpr int kFpBatch = 64 / kFpCols;  // slabs per batch (one lane per slab and column)
 constexpr int kColRows = 63;            // detector rows per wavefront (64 lanes = its 64 row edges)
 #ifndef CTR_FP_WAVES
@@ -276,15 +281,15 @@ constexpr int kColRows = 63;            // detector rows per wavefront (64 lanes
 #endif
 constexpr int kFpWaves = CTR_FP_WAVES;  // wavefronts per workgroup: 16 adjacent columns, 64 bytes of a row
 #ifndef CTR_FP_MIN_WAVES
-#define CTR_FP_MIN_WAVES 5  // resident wavefronts per SIMD the register budget must allow
+#define CTR_FP_MIN_WAVES 3  // resident wavefronts per SIMD the register budget must allow
 #endif
-constexpr int kFpMaxLoads = 3;          // segment loads per slab
+constexpr int kFpMaxLoads = 4;          // segment loads per slab
 constexpr int kFpMaxQuads = 64 * kFpMaxLoads;
-// Views per block of the workgroup order: consecutive workgroups on one XCD take kViewBlock
-// adjacent views of one column group, then the XCD's next column group, so the workgroups
+// View pairs per block of the workgroup order: consecutive workgroups on one XCD take kViewBlock
+// adjacent view pairs of one column group, then the XCD's next column group, so the workgroups
 // resident on an XCD at one time trace nearby rays and share volume lines in its L2.
 #ifndef CTR_FP_VIEW_BLOCK
-#define CTR_FP_VIEW_BLOCK 64
+#define CTR_FP_VIEW_BLOCK 32
 #endif
 constexpr int kViewBlock = CTR_FP_VIEW_BLOCK;
 #ifndef CTR_FP_XCD_RUN
@@ -315,7 +320,7 @@ __device__ inline int quad_max(int v) {
     return max(v, quad_xor2(v));
 }
 
-// per (slab, column) record of a batch: packed flags (see below) and the weights
+// per (slab, column, view) record of a batch: packed flags (see below) and the weights
 constexpr int kRecHit = 1 << 10;
 
 __global__ void __launch_bounds__(64 * kFpWaves, CTR_FP_MIN_WAVES) dd_forward_kernel(
@@ -326,7 +331,7 @@ __global__ void __launch_bounds__(64 * kFpWaves, CTR_FP_MIN_WAVES) dd_forward_ke
     // workgroup b runs on XCD b % 8, which takes the runs of kXcdRun column groups
     // cg = (8 c + xcd) kXcdRun + m: every XCD gets an equal share of every band of detector
     // rows (the rows outside the volume's shadow are cheap).  On one XCD the local index runs
-    // over (row block, view block, c, view in block, m), fastest last.
+    // over (row block, view-pair block, c, pair in block, m), fastest last.
     const int b = blockIdx.x;
     const int xcd = b & 7, l = b >> 3;
     const int runs = (col_groups + 8 * kXcdRun - 1) / (8 * kXcdRun);
@@ -338,9 +343,10 @@ __global__ void __launch_bounds__(64 * kFpWaves, CTR_FP_MIN_WAVES) dd_forward_ke
     const int r1 = l2 / kViewBlock;
     const int cg = ((r1 % runs) * 8 + xcd) * kXcdRun + m;
     const int r2 = r1 / runs;
-    const int view = (r2 % n_vblocks) * kViewBlock + vi;
+    const int vp = (r2 % n_vblocks) * kViewBlock + vi;  // view pair
     const int rb = r2 / n_vblocks;
-    if (view >= n_views || cg >= col_groups) return;
+    const int view0 = kFpViews * vp;
+    if (view0 >= n_views || cg >= col_groups) return;
     const int lane = threadIdx.x & 63;
     // wave-uniform: lets the compiler keep column data and addresses in scalar registers
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
@@ -350,303 +356,365 @@ __global__ void __launch_bounds__(64 * kFpWaves, CTR_FP_MIN_WAVES) dd_forward_ke
     const int k = e0 + lane;  // this lane's row (lanes 0 .. kColRows-1) and edge
     const bool active = lane < kColRows && k < nv;
     const int kk = min(k, nv - 1);
-    const int br = branch[view];
-    float* __restrict__ out_row = proj + ((size_t)view * nv + kk) * nu + j0;
     const bool vec_store = (nu & 3) == 0 && j0 + kFpCols <= nu && (reinterpret_cast<uintptr_t>(proj) & 15) == 0;
-    if (br != CTR_BRANCH_X && br != CTR_BRANCH_Y) {
-        if (active) {
+    // the pair's views: present, slab orientation (x or y slabs; z slabs write zeros)
+    int br[kFpViews];
+    bool use[kFpViews];
+#pragma unroll
+    for (int v = 0; v < kFpViews; ++v) {
+        const bool present = view0 + v < n_views;
+        br[v] = present ? branch[view0 + v] : -1;
+        use[v] = br[v] == CTR_BRANCH_X || br[v] == CTR_BRANCH_Y;
+        if (present && !use[v] && active) {
+            float* o = proj + ((size_t)(view0 + v) * nv + kk) * nu + j0;
 #pragma unroll
             for (int c = 0; c < kFpCols; ++c)
-                if (j0 + c < nu) out_row[c] = 0.0f;
+                if (j0 + c < nu) o[c] = 0.0f;
         }
-        return;
     }
-    const bool xs = (br == CTR_BRANCH_X);
-    const float* __restrict__ src = xs ? t_xs : t_ys;
-    const int ns = xs ? nx : ny;
-    const int nt = xs ? ny : nx;
-    // byte offsets inside a batch of slabs fit in 31 bits (host check)
-    const int line_bytes = 4 * nzp;
-    const int slab_bytes = line_bytes * nt;
-    const char* __restrict__ src_bytes = reinterpret_cast<const char*>(src);
-    const float* T = tabs + (size_t)view * stride;
-    const float gam = T[6 * nu + 2];
-    const float nue = rows[min(k, nv)];  // this lane's edge slope
-    const float fnz = (float)nz, fnt = (float)nt;
+    if (!use[0] && !use[1]) return;
+    // walks over the slabs: both views at once when they share the slab orientation, else one
+    // view per walk (view mask bit v)
+    int walks[kFpViews], n_walks = 0;
+    if (use[0] && use[1] && br[0] == br[1]) {
+        walks[n_walks++] = 3;
+    } else {
+        if (use[0]) walks[n_walks++] = 1;
+        if (use[1]) walks[n_walks++] = 2;
+    }
+    const float fnz = (float)nz;
     // z extent of the wavefront's edges
+    const float nue = rows[min(k, nv)];  // this lane's edge slope
     const float nu_first = rows[e0], nu_last = rows[min(e0 + kColRows, nv)];
-    // gamma in a vector register: the edge fma then reads one scalar operand (tt)
-    float gam_v;
-    asm volatile("v_mov_b32 %0, %1" : "=v"(gam_v) : "s"(gam));
     // as (slab, column) worker: column j0 + cl of slab n0 + (lane >> 2)
     const int cl = lane & (kFpCols - 1);
     const int jc = min(j0 + cl, nu - 1);
     const bool col_ok = j0 + cl < nu;
-    const float a_al = T[jc], b_al = T[jc + 1];
-    const float a_be = T[nu + 1 + jc], b_be = T[nu + 2 + jc];
-    const float T0 = T[2 * nu + 2 + jc], dT = T[3 * nu + 2 + jc];
-    double H[kFpCols];
+    // per view: gamma (scalar, and in a vector register so that the edge fma reads one scalar
+    // operand) and the lane's column's footprint lines and slab fractions
+    const float* T[kFpViews];
+    float gam[kFpViews], gam_v[kFpViews];
+    float a_al[kFpViews], b_al[kFpViews], a_be[kFpViews], b_be[kFpViews], T0[kFpViews], dT[kFpViews];
 #pragma unroll
-    for (int c = 0; c < kFpCols; ++c) H[c] = 0.0;
+    for (int v = 0; v < kFpViews; ++v) {
+        T[v] = tabs + (size_t)min(view0 + v, n_views - 1) * stride;
+        gam[v] = T[v][6 * nu + 2];
+        asm volatile("v_mov_b32 %0, %1" : "=v"(gam_v[v]) : "s"(gam[v]));
+        a_al[v] = T[v][jc];
+        b_al[v] = T[v][jc + 1];
+        a_be[v] = T[v][nu + 1 + jc];
+        b_be[v] = T[v][nu + 2 + jc];
+        T0[v] = T[v][2 * nu + 2 + jc];
+        dT[v] = T[v][3 * nu + 2 + jc];
+    }
+    double H[kFpViews][kFpCols];
+#pragma unroll
+    for (int v = 0; v < kFpViews; ++v)
+#pragma unroll
+        for (int c = 0; c < kFpCols; ++c) H[v][c] = 0.0;
     __shared__ uint4 segs[kFpWaves][kFpMaxQuads];
-    __shared__ float4 recs[kFpWaves][64];
+    __shared__ float4 recs[kFpWaves][kFpViews][64];
     uint4* __restrict__ seg = segs[wave];
     const char* segs_bytes = reinterpret_cast<const char*>(&segs[0][0]);
     const int seg_off = wave * kFpMaxQuads * 16;  // this wavefront's segment buffer in segs
-    const float4* rec = recs[wave];
-    // this lane's edge at the slab with fraction tt: its table entry's byte offset in a line
-    // and the fractional part of the clamped edge
-    auto edge = [&](float tt, unsigned& off, float& fr) {
-        const float c = __builtin_amdgcn_fmed3f(fmaf(nue, tt, gam_v), 0.0f, fnz);
-        const float fl = floorf(c);
-        fr = c - fl;
-        off = 4u * (unsigned)fl;
-    };
     // lanes g = 64 i + lane of segment load i: (g + 1/2) as a float, for the line index
     float gh[kFpMaxLoads];
 #pragma unroll
     for (int i = 0; i < kFpMaxLoads; ++i) gh[i] = (float)(64 * i + lane) + 0.5f;
-    // Can any slab hit?  The footprint lines and tt are affine in the slab index, so their
-    // values at the first and last slab bound every slab's; a wavefront whose four columns
-    // miss the volume (transversely, or in z for all of its rows) writes zeros.
-    bool may_hit = false;
-    {
-        const float fN = (float)(ns - 1);
-        const float a1N = fmaf(a_be, fN, a_al), a2N = fmaf(b_be, fN, b_al);
-        const float lo = fminf(fminf(a_al, b_al), fminf(a1N, a2N));
-        const float hi = fmaxf(fmaxf(a_al, b_al), fmaxf(a1N, a2N));
-        const float ttN = fmaf(dT, fN, T0);
-        const float z1 = fmaf(nu_first, T0, gam), z2 = fmaf(nu_last, T0, gam);
-        const float z3 = fmaf(nu_first, ttN, gam), z4 = fmaf(nu_last, ttN, gam);
-        const float zmin = fminf(fminf(z1, z2), fminf(z3, z4)), zmax = fmaxf(fmaxf(z1, z2), fmaxf(z3, z4));
-        may_hit = col_ok && hi >= -1.0f && lo <= fnt + 1.0f && zmax >= -1.0f && zmin <= fnz + 1.0f;
-    }
-    const int n_end = __ballot(may_hit) ? ns : 0;
-    for (int n0 = 0; n0 < n_end; n0 += kFpBatch) {
-        // the batch's slabs through one buffer resource: a line is addressed by a scalar byte
-        // offset, the lane's quad by a vector one
-        const char* batch = src_bytes + (size_t)n0 * slab_bytes;
-        const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-            (void*)batch, 0, (int)((size_t)min(kFpBatch, ns - n0) * slab_bytes), 0x00020000);
-        // lane 4 s + c: slab n0 + s, column j0 + c.  Packed record p: first line relative to
-        // the slab's union (bits 0-7), hit (bit 10).  Union u1: first line (bits 0-23), lines
-        // staged (24-31); u2: first z breakpoint (0-15), quads per line (16-23), walk (24-27:
-        // 0 no column hits, 1-3 staged with that many segment loads and two lines per column,
-        // 4-6 the same with three lines per column, 7 explicit).
-        int p = 0, u1 = 0, u2 = 0;
-        {
-            const int n = n0 + (lane >> 2);
-            float tt = 1.0f, w0 = 0.0f, w1 = 0.0f, w2 = 0.0f;
-            int kas = 0x7fffffff, kend2 = -1, kend3 = -1, zlo = 0x7fffffff, zhi = -1, da = -1;
-            if (col_ok && n < ns) {
-                const float fn = (float)n;
-                const float a1 = fmaf(a_be, fn, a_al), a2 = fmaf(b_be, fn, b_al);
-                const float alo = fminf(a1, a2), ahi = fmaxf(a1, a2);
-                const int ks = max((int)floorf(alo), 0), ke = min((int)floorf(ahi), nt - 1);
-                tt = fmaf(dT, fn, T0);
-                const float za = fmaf(nu_first, tt, gam), zb = fmaf(nu_last, tt, gam);
-                const bool z_hit = fmaxf(za, zb) >= -0.01f && fminf(za, zb) <= fnz + 0.01f;
-                if (ks <= ke && z_hit) {
-                    da = ke - ks;
-                    const float wa = ahi - alo;
-                    // overlaps with cells ks .. ks+2 (floor(max(alo, 0)) == ks); negative
-                    // values past the footprint fail the threshold test
-                    const float lo = fmaxf(alo, 0.0f), hi = fminf(ahi, fnt);
-                    const float s1 = (float)ks + 1.0f;
-                    float o0 = fminf(hi, s1) - lo, o1 = fminf(hi, s1 + 1.0f) - s1, o2 = hi - (s1 + 1.0f);
-                    const float tl = zero_tol * wa;
-                    o0 = (o0 >= tl) ? o0 : 0.0f;
-                    o1 = (o1 >= tl) ? o1 : 0.0f;
-                    o2 = (o2 >= tl) ? o2 : 0.0f;
-                    const float r = __builtin_amdgcn_rcpf(wa * tt);
-                    w0 = o0 * r;
-                    w1 = o1 * r;
-                    w2 = o2 * r;
-                    // breakpoints of the wavefront's edges: floor of the clamped end edges, the
-                    // same arithmetic as each lane's (the edges are monotone in the lane)
-                    const int kf = (int)floorf(__builtin_amdgcn_fmed3f(fmaf(nu_first, tt, gam), 0.0f, fnz));
-                    const int kl = (int)floorf(__builtin_amdgcn_fmed3f(fmaf(nu_last, tt, gam), 0.0f, fnz));
-                    kas = ks;
-                    kend2 = ks + 1;
-                    kend3 = ks + 2;
-                    zlo = min(kf, kl);
-                    zhi = max(kf, kl);
-                }
-            }
-            recs[wave][lane] = make_float4(tt, w0, w1, w2);
-            const int maxda = quad_max(da);
-            const int L0 = quad_min(kas);
-            // every column reads two (three) consecutive lines from its first one; lines past
-            // its footprint have weight 0 but are staged, so that every read is of staged data
-            const int L1 = (maxda >= 2) ? quad_max(kend3) : quad_max(kend2);
-            const int zl = quad_min(zlo), zh = quad_max(zhi);
-            if (maxda >= 0) {
-                // the segment starts at a multiple of 4 breakpoints (16-byte aligned loads) and
-                // holds breakpoints kseg .. zh + 1; past the end of a line it reads the next one
-                // (or the table's slack, or nothing past the batch), which no weight uses
-                const int kseg = zl & ~3;
-                const int nq = (zh - kseg + 5) >> 2;
-                const int nlines = L1 - L0 + 1;
-                const int nl = (nlines * nq + 63) >> 6;
-                const bool staged = maxda <= 2 && nl <= kFpMaxLoads;
-                const int walk = !staged ? 7 : (maxda >= 2 ? 3 + nl : nl);
-                u1 = staged ? (L0 | (nlines << 24)) : 0;
-                u2 = kseg | (min(nq, 255) << 16) | (walk << 24);  // kseg <= nz < 2^16
-                // staged: kas - L0 < nlines <= kFpMaxQuads fits its 8 bits
-                if (da >= 0) p = (staged ? kas - L0 : 0) | kRecHit;
-            }
+    for (int w = 0; w < n_walks; ++w) {
+        const int vmask = walks[w];
+        const int brw = (vmask & 1) ? br[0] : br[1];
+        const bool xs = (brw == CTR_BRANCH_X);
+        const float* __restrict__ src = xs ? t_xs : t_ys;
+        const int ns = xs ? nx : ny;
+        const int nt = xs ? ny : nx;
+        const float fnt = (float)nt;
+        // byte offsets inside a batch of slabs fit in 31 bits (host check)
+        const int line_bytes = 4 * nzp;
+        const int slab_bytes = line_bytes * nt;
+        const char* __restrict__ src_bytes = reinterpret_cast<const char*>(src);
+        // Can any slab hit?  The footprint lines and tt are affine in the slab index, so their
+        // values at the first and last slab bound every slab's; a wavefront whose columns miss
+        // the volume (transversely, or in z for all of its rows) in every view of the walk
+        // only writes zeros.
+        bool may_hit = false;
+#pragma unroll
+        for (int v = 0; v < kFpViews; ++v) {
+            if (!(vmask & (1 << v))) continue;
+            const float fN = (float)(ns - 1);
+            const float a1N = fmaf(a_be[v], fN, a_al[v]), a2N = fmaf(b_be[v], fN, b_al[v]);
+            const float lo = fminf(fminf(a_al[v], b_al[v]), fminf(a1N, a2N));
+            const float hi = fmaxf(fmaxf(a_al[v], b_al[v]), fmaxf(a1N, a2N));
+            const float ttN = fmaf(dT[v], fN, T0[v]);
+            const float z1 = fmaf(nu_first, T0[v], gam[v]), z2 = fmaf(nu_last, T0[v], gam[v]);
+            const float z3 = fmaf(nu_first, ttN, gam[v]), z4 = fmaf(nu_last, ttN, gam[v]);
+            const float zmin = fminf(fminf(z1, z2), fminf(z3, z4)), zmax = fmaxf(fmaxf(z1, z2), fmaxf(z3, z4));
+            may_hit |= col_ok && hi >= -1.0f && lo <= fnt + 1.0f && zmax >= -1.0f && zmin <= fnz + 1.0f;
         }
-        const uint64_t heads = __ballot(cl == 0);
-        // staged slabs, software-pipelined in pairs: both slabs' segment loads go out, then both
-        // are summed; no load result lives across rounds, so nothing waits to be copied
-        auto walk = [&](auto nl_c, auto lines_c, uint64_t mask) {
-            constexpr int NL = decltype(nl_c)::value;        // segment loads per slab
-            constexpr int LINES = decltype(lines_c)::value;  // lines read per column
-            struct Set {
-                v4u d[NL];
-                int q;
-            };
-            auto issue = [&](int q, Set& S) {
-                S.q = q;
-                const int U1 = __builtin_amdgcn_readlane(u1, 4 * q), U2 = __builtin_amdgcn_readlane(u2, 4 * q);
-                const int L0 = U1 & 0xffffff, nlines = (unsigned)U1 >> 24;
-                const int kseg = U2 & 0xffff, nq = (U2 >> 16) & 0xff;
-                // quad g of the segment: line g / nq (the quotient from an FP32 reciprocal:
-                // (g + 1/2)/nq is at least 1/(2 nq) from an integer and its rounding error below
-                // 1e-4, so the floor is exact for g < 192, nq < 256), quad g - line nq, at byte
-                // line * line_bytes + 16 quad = 16 g + line (line_bytes - 16 nq)
-                const float inv_nq = __builtin_amdgcn_rcpf((float)nq);
-                const int step = line_bytes - 16 * nq;
-                const int soff = q * slab_bytes + L0 * line_bytes + 4 * kseg;
+        const int n_end = __ballot(may_hit) ? ns : 0;
+        for (int n0 = 0; n0 < n_end; n0 += kFpBatch) {
+            // the batch's slabs through one buffer resource: a line is addressed by a scalar byte
+            // offset, the lane's quad by a vector one
+            const char* batch = src_bytes + (size_t)n0 * slab_bytes;
+            const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+                (void*)batch, 0, (int)((size_t)min(kFpBatch, ns - n0) * slab_bytes), 0x00020000);
+            // lane 4 s + c: slab n0 + s, column j0 + c.  Packed record p[v] of view v: first line
+            // relative to the slab's union (bits 0-7), hit (bit 10).  Union u1: first line (bits
+            // 0-23), lines staged (24-31); u2: first z breakpoint (0-15), quads per line (16-23),
+            // walk (24-27: 0 no column-view hits, 1-4 staged with that many segment loads and two
+            // lines per column, 5-8 the same with three lines per column, 9 explicit).
+            int p[kFpViews];
+            int u1 = 0, u2 = 0;
+            {
+                const int n = n0 + (lane >> 2);
+                const float fn = (float)n;
+                int kas_u = 0x7fffffff, kend_u = -1, zlo_u = 0x7fffffff, zhi_u = -1, da_u = -1;
+                int kas_v[kFpViews], da_v[kFpViews];
 #pragma unroll
-                for (int i = 0; i < NL; ++i) {
-                    const int ln = (int)(gh[i] * inv_nq);
-                    // quads past the segment get an offset beyond the buffer: the range check
-                    // returns 0 for them without a memory access
-                    const int voff = ln < nlines ? 16 * (64 * i + lane) + __mul24(ln, step) : 0x7ffffff0;
-                    S.d[i] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, voff, soff, 0);
+                for (int v = 0; v < kFpViews; ++v) {
+                    float tt = 1.0f, w0 = 0.0f, w1 = 0.0f, w2 = 0.0f;
+                    int kas = 0x7fffffff, da = -1;
+                    if ((vmask & (1 << v)) && col_ok && n < ns) {
+                        const float a1 = fmaf(a_be[v], fn, a_al[v]), a2 = fmaf(b_be[v], fn, b_al[v]);
+                        const float alo = fminf(a1, a2), ahi = fmaxf(a1, a2);
+                        const int ks = max((int)floorf(alo), 0), ke = min((int)floorf(ahi), nt - 1);
+                        tt = fmaf(dT[v], fn, T0[v]);
+                        const float za = fmaf(nu_first, tt, gam[v]), zb = fmaf(nu_last, tt, gam[v]);
+                        const bool z_hit = fmaxf(za, zb) >= -0.01f && fminf(za, zb) <= fnz + 0.01f;
+                        if (ks <= ke && z_hit) {
+                            da = ke - ks;
+                            const float wa = ahi - alo;
+                            // overlaps with cells ks .. ks+2 (floor(max(alo, 0)) == ks); negative
+                            // values past the footprint fail the threshold test
+                            const float lo = fmaxf(alo, 0.0f), hi = fminf(ahi, fnt);
+                            const float s1 = (float)ks + 1.0f;
+                            float o0 = fminf(hi, s1) - lo, o1 = fminf(hi, s1 + 1.0f) - s1, o2 = hi - (s1 + 1.0f);
+                            const float tl = zero_tol * wa;
+                            o0 = (o0 >= tl) ? o0 : 0.0f;
+                            o1 = (o1 >= tl) ? o1 : 0.0f;
+                            o2 = (o2 >= tl) ? o2 : 0.0f;
+                            const float r = __builtin_amdgcn_rcpf(wa * tt);
+                            w0 = o0 * r;
+                            w1 = o1 * r;
+                            w2 = o2 * r;
+                            // breakpoints of the wavefront's edges: floor of the clamped end
+                            // edges, the same arithmetic as each lane's (the edges are monotone
+                            // in the lane)
+                            const int kf = (int)floorf(__builtin_amdgcn_fmed3f(za, 0.0f, fnz));
+                            const int kl = (int)floorf(__builtin_amdgcn_fmed3f(zb, 0.0f, fnz));
+                            kas = ks;
+                            kas_u = min(kas_u, ks);
+                            // every column reads two (three) consecutive lines from its first
+                            // one; lines past its footprint have weight 0 but are staged, so
+                            // that every read is of staged data
+                            kend_u = max(kend_u, ks + 2);
+                            zlo_u = min(zlo_u, min(kf, kl));
+                            zhi_u = max(zhi_u, max(kf, kl));
+                            da_u = max(da_u, da);
+                        }
+                    }
+                    recs[wave][v][lane] = make_float4(tt, w0, w1, w2);
+                    kas_v[v] = kas;
+                    da_v[v] = da;
                 }
-            };
-            auto consume = [&](const Set& S) {
+                const int maxda = quad_max(da_u);
+                const int L0 = quad_min(kas_u);
+                // two lines per column unless a footprint covers three cells
+                const int L1 = quad_max(kend_u) - (maxda >= 2 ? 0 : 1);
+                const int zl = quad_min(zlo_u), zh = quad_max(zhi_u);
 #pragma unroll
-                for (int i = 0; i < NL; ++i) seg[64 * i + lane] = make_uint4(S.d[i].x, S.d[i].y, S.d[i].z, S.d[i].w);
-                const int U2 = __builtin_amdgcn_readlane(u2, 4 * S.q);
-                const int kseg = U2 & 0xffff, nq16 = (U2 >> 12) & 0xff0;
-                float4 r[kFpCols];
+                for (int v = 0; v < kFpViews; ++v) p[v] = 0;
+                if (maxda >= 0) {
+                    // the segment starts at a multiple of 4 breakpoints (16-byte aligned loads)
+                    // and holds breakpoints kseg .. zh + 1; past the end of a line it reads the
+                    // next one (or the table's slack, or nothing past the batch), which no weight
+                    // uses
+                    const int kseg = zl & ~3;
+                    const int nq = (zh - kseg + 5) >> 2;
+                    const int nlines = L1 - L0 + 1;
+                    const int nl = (nlines * nq + 63) >> 6;
+                    const bool staged = maxda <= 2 && nl <= kFpMaxLoads;
+                    const int walk = !staged ? 9 : (maxda >= 2 ? 4 + nl : nl);
+                    u1 = staged ? (L0 | (nlines << 24)) : 0;
+                    u2 = kseg | (min(nq, 255) << 16) | (walk << 24);  // kseg <= nz < 2^16
+                    // staged: kas - L0 < nlines <= kFpMaxQuads fits its 8 bits
 #pragma unroll
-                for (int c = 0; c < kFpCols; ++c) r[c] = rec[4 * S.q + c];
-                unsigned lb[kFpCols];
-                float fr[kFpCols];
-#pragma unroll
-                for (int c = 0; c < kFpCols; ++c) {
-                    const int pc = __builtin_amdgcn_readlane(p, 4 * S.q + c);
-                    // byte of the column's first line in the segment buffers, less the segment's
-                    // first breakpoint (scalar): one shift-add per lane
-                    const unsigned sbase = (unsigned)(seg_off + (pc & 0xff) * nq16 - 4 * kseg);
-                    const float cz = __builtin_amdgcn_fmed3f(fmaf(nue, r[c].x, gam_v), 0.0f, fnz);
-                    const float fl = floorf(cz);
-                    fr[c] = cz - fl;
-                    // a column that misses this slab reads the segment's start with weight 0
-                    lb[c] = (pc & kRecHit) ? 4u * (unsigned)fl + sbase : (unsigned)seg_off;
-                }
-                PTaps t[kFpCols][LINES];
-#pragma unroll
-                for (int c = 0; c < kFpCols; ++c)
-#pragma unroll
-                    for (int a = 0; a < LINES; ++a)
-                        t[c][a] = *reinterpret_cast<const PTaps*>(segs_bytes + lb[c] + a * nq16);
-#pragma unroll
-                for (int c = 0; c < kFpCols; ++c) {
-                    const float w[3] = {r[c].y, r[c].z, r[c].w};
-                    float sum = w[0] * fmaf(fr[c], t[c][0].b - t[c][0].a, t[c][0].a);
-#pragma unroll
-                    for (int a = 1; a < LINES; ++a)
-                        sum = fmaf(w[a], fmaf(fr[c], t[c][a].b - t[c][a].a, t[c][a].a), sum);
-                    H[c] += (double)sum;
-                }
-            };
-            // the rare walks (three segment loads, or three lines per column) go one slab at a
-            // time, which keeps their registers within the common walks' budget
-            constexpr bool kPair = NL <= 2 && LINES == 2;
-            while (mask) {
-                Set S0, S1;
-                const int q0 = __builtin_ctzll(mask) >> 2;
-                mask &= mask - 1;
-                issue(q0, S0);
-                if (kPair && mask) {
-                    const int q1 = __builtin_ctzll(mask) >> 2;
-                    mask &= mask - 1;
-                    issue(q1, S1);
-                    __builtin_amdgcn_sched_barrier(0);
-                    consume(S0);
-                    consume(S1);
-                } else {
-                    consume(S0);
+                    for (int v = 0; v < kFpViews; ++v)
+                        if (da_v[v] >= 0) p[v] = (staged ? kas_v[v] - L0 : 0) | kRecHit;
                 }
             }
-        };
-        const int wk = (u2 >> 24) & 7;
-        using I1 = std::integral_constant<int, 1>;
-        using I2 = std::integral_constant<int, 2>;
-        using I3 = std::integral_constant<int, 3>;
-        uint64_t m;
-        if ((m = __ballot(wk == 1) & heads)) walk(I1{}, I2{}, m);
-        if ((m = __ballot(wk == 2) & heads)) walk(I2{}, I2{}, m);
-        if ((m = __ballot(wk == 3) & heads)) walk(I3{}, I2{}, m);
-        if ((m = __ballot(wk == 4) & heads)) walk(I1{}, I3{}, m);
-        if ((m = __ballot(wk == 5) & heads)) walk(I2{}, I3{}, m);
-        if ((m = __ballot(wk == 6) & heads)) walk(I3{}, I3{}, m);
-        // explicit slabs: every transverse cell of each column's footprint, pairs read from
-        // memory
-        uint64_t m2 = __ballot(wk == 7) & heads;
-        while (m2) {
-            const int q = __builtin_ctzll(m2) >> 2;
-            m2 &= m2 - 1;
-            const float fn = (float)(n0 + q);
+            const uint64_t heads = __ballot(cl == 0);
+            // staged slabs, software-pipelined in pairs: both slabs' segment loads go out, then
+            // both are summed; no load result lives across rounds, so nothing waits to be copied
+            auto walk_slabs = [&](auto nl_c, auto lines_c, uint64_t mask) {
+                constexpr int NL = decltype(nl_c)::value;        // segment loads per slab
+                constexpr int LINES = decltype(lines_c)::value;  // lines read per column
+                struct Set {
+                    v4u d[NL];
+                    int q;
+                };
+                auto issue = [&](int q, Set& S) {
+                    S.q = q;
+                    const int U1 = __builtin_amdgcn_readlane(u1, 4 * q), U2 = __builtin_amdgcn_readlane(u2, 4 * q);
+                    const int L0 = U1 & 0xffffff, nlines = (unsigned)U1 >> 24;
+                    const int kseg = U2 & 0xffff, nq = (U2 >> 16) & 0xff;
+                    // quad g of the segment: line g / nq (the quotient from an FP32 reciprocal:
+                    // (g + 1/2)/nq is at least 1/(2 nq) from an integer and its rounding error
+                    // below 1e-4, so the floor is exact for g < 256, nq < 256), quad g - line nq,
+                    // at byte line * line_bytes + 16 quad = 16 g + line (line_bytes - 16 nq)
+                    const float inv_nq = __builtin_amdgcn_rcpf((float)nq);
+                    const int step = line_bytes - 16 * nq;
+                    const int soff = q * slab_bytes + L0 * line_bytes + 4 * kseg;
 #pragma unroll
-            for (int c = 0; c < kFpCols; ++c) {
-                const int pc = __builtin_amdgcn_readlane(p, 4 * q + c);
-                if (!(pc & kRecHit)) continue;
-                const float4 r = rec[4 * q + c];
-                unsigned off;
-                float fr;
-                edge(r.x, off, fr);
-                const float a1 = fmaf(lane_f(a_be, c), fn, lane_f(a_al, c));
-                const float a2 = fmaf(lane_f(b_be, c), fn, lane_f(b_al, c));
-                const float alo = fminf(a1, a2), ahi = fmaxf(a1, a2);
-                const float wa = ahi - alo;
-                const int kas = max((int)floorf(alo), 0), kae = min((int)floorf(ahi), nt - 1);
-                float sum = 0.0f;
-                for (int ka = kas; ka <= kae; ++ka) {
-                    const float ov = fminf(ahi, (float)(ka + 1)) - fmaxf(alo, (float)ka);
-                    if (ov < zero_tol * wa) continue;
-                    const v2u v = __builtin_amdgcn_raw_buffer_load_b64(rsrc, (int)off, q * slab_bytes + ka * line_bytes, 0);
-                    const float pa = __uint_as_float(v.x), pb = __uint_as_float(v.y);
-                    sum = fmaf(ov, fmaf(fr, pb - pa, pa), sum);
+                    for (int i = 0; i < NL; ++i) {
+                        const int ln = (int)(gh[i] * inv_nq);
+                        // quads past the segment get an offset beyond the buffer: the range
+                        // check returns 0 for them without a memory access
+                        const int voff = ln < nlines ? 16 * (64 * i + lane) + __mul24(ln, step) : 0x7ffffff0;
+                        S.d[i] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, voff, soff, 0);
+                    }
+                };
+                auto consume = [&](const Set& S) {
+#pragma unroll
+                    for (int i = 0; i < NL; ++i)
+                        seg[64 * i + lane] = make_uint4(S.d[i].x, S.d[i].y, S.d[i].z, S.d[i].w);
+                    const int U2 = __builtin_amdgcn_readlane(u2, 4 * S.q);
+                    const int kseg = U2 & 0xffff, nq16 = (U2 >> 12) & 0xff0;
+#pragma unroll
+                    for (int v = 0; v < kFpViews; ++v) {
+                        if (!(vmask & (1 << v))) continue;
+                        const float4* rec = recs[wave][v];
+                        float4 r[kFpCols];
+#pragma unroll
+                        for (int c = 0; c < kFpCols; ++c) r[c] = rec[4 * S.q + c];
+                        unsigned lb[kFpCols];
+                        float fr[kFpCols];
+#pragma unroll
+                        for (int c = 0; c < kFpCols; ++c) {
+                            const int pc = __builtin_amdgcn_readlane(p[v], 4 * S.q + c);
+                            // byte of the column's first line in the segment buffers, less the
+                            // segment's first breakpoint (scalar): one shift-add per lane
+                            const unsigned sbase = (unsigned)(seg_off + (pc & 0xff) * nq16 - 4 * kseg);
+                            const float cz = __builtin_amdgcn_fmed3f(fmaf(nue, r[c].x, gam_v[v]), 0.0f, fnz);
+                            const float fl = floorf(cz);
+                            fr[c] = cz - fl;
+                            // a column that misses this slab reads the segment's start with
+                            // weight 0
+                            lb[c] = (pc & kRecHit) ? 4u * (unsigned)fl + sbase : (unsigned)seg_off;
+                        }
+                        PTaps t[kFpCols][LINES];
+#pragma unroll
+                        for (int c = 0; c < kFpCols; ++c)
+#pragma unroll
+                            for (int a = 0; a < LINES; ++a)
+                                t[c][a] = *reinterpret_cast<const PTaps*>(segs_bytes + lb[c] + a * nq16);
+#pragma unroll
+                        for (int c = 0; c < kFpCols; ++c) {
+                            const float wgt[3] = {r[c].y, r[c].z, r[c].w};
+                            float sum = wgt[0] * fmaf(fr[c], t[c][0].b - t[c][0].a, t[c][0].a);
+#pragma unroll
+                            for (int a = 1; a < LINES; ++a)
+                                sum = fmaf(wgt[a], fmaf(fr[c], t[c][a].b - t[c][a].a, t[c][a].a), sum);
+                            H[v][c] += (double)sum;
+                        }
+                    }
+                };
+                // the rare walks (three or four segment loads, or three lines per column) go one
+                // slab at a time, which keeps their registers within the common walks' budget
+                constexpr bool kPair = NL <= 2 && LINES == 2;
+                while (mask) {
+                    Set S0, S1;
+                    const int q0 = __builtin_ctzll(mask) >> 2;
+                    mask &= mask - 1;
+                    issue(q0, S0);
+                    if (kPair && mask) {
+                        const int q1 = __builtin_ctzll(mask) >> 2;
+                        mask &= mask - 1;
+                        issue(q1, S1);
+                        __builtin_amdgcn_sched_barrier(0);
+                        consume(S0);
+                        consume(S1);
+                    } else {
+                        consume(S0);
+                    }
                 }
-                H[c] += (double)(sum * __builtin_amdgcn_rcpf(wa * r.x));
+            };
+            const int wk = (u2 >> 24) & 15;
+            using I1 = std::integral_constant<int, 1>;
+            using I2 = std::integral_constant<int, 2>;
+            using I3 = std::integral_constant<int, 3>;
+            using I4 = std::integral_constant<int, 4>;
+            uint64_t mk;
+            if ((mk = __ballot(wk == 1) & heads)) walk_slabs(I1{}, I2{}, mk);
+            if ((mk = __ballot(wk == 2) & heads)) walk_slabs(I2{}, I2{}, mk);
+            if ((mk = __ballot(wk == 3) & heads)) walk_slabs(I3{}, I2{}, mk);
+            if ((mk = __ballot(wk == 4) & heads)) walk_slabs(I4{}, I2{}, mk);
+            if ((mk = __ballot(wk == 5) & heads)) walk_slabs(I1{}, I3{}, mk);
+            if ((mk = __ballot(wk == 6) & heads)) walk_slabs(I2{}, I3{}, mk);
+            if ((mk = __ballot(wk == 7) & heads)) walk_slabs(I3{}, I3{}, mk);
+            if ((mk = __ballot(wk == 8) & heads)) walk_slabs(I4{}, I3{}, mk);
+            // explicit slabs: every transverse cell of each column-view's footprint, pairs read
+            // from memory
+            uint64_t m2 = __ballot(wk == 9) & heads;
+            while (m2) {
+                const int q = __builtin_ctzll(m2) >> 2;
+                m2 &= m2 - 1;
+                const float fn = (float)(n0 + q);
+#pragma unroll
+                for (int v = 0; v < kFpViews; ++v) {
+                    if (!(vmask & (1 << v))) continue;
+#pragma unroll
+                    for (int c = 0; c < kFpCols; ++c) {
+                        const int pc = __builtin_amdgcn_readlane(p[v], 4 * q + c);
+                        if (!(pc & kRecHit)) continue;
+                        const float4 r = recs[wave][v][4 * q + c];
+                        const float cz = __builtin_amdgcn_fmed3f(fmaf(nue, r.x, gam_v[v]), 0.0f, fnz);
+                        const float fl = floorf(cz);
+                        const float fr = cz - fl;
+                        const int off = 4 * (int)fl;
+                        const float a1 = fmaf(lane_f(a_be[v], c), fn, lane_f(a_al[v], c));
+                        const float a2 = fmaf(lane_f(b_be[v], c), fn, lane_f(b_al[v], c));
+                        const float alo = fminf(a1, a2), ahi = fmaxf(a1, a2);
+                        const float wa = ahi - alo;
+                        const int kas = max((int)floorf(alo), 0), kae = min((int)floorf(ahi), nt - 1);
+                        float sum = 0.0f;
+                        for (int ka = kas; ka <= kae; ++ka) {
+                            const float ov = fminf(ahi, (float)(ka + 1)) - fmaxf(alo, (float)ka);
+                            if (ov < zero_tol * wa) continue;
+                            const v2u pv = __builtin_amdgcn_raw_buffer_load_b64(rsrc, off, q * slab_bytes + ka * line_bytes, 0);
+                            const float pa = __uint_as_float(pv.x), pb = __uint_as_float(pv.y);
+                            sum = fmaf(ov, fmaf(fr, pb - pa, pa), sum);
+                        }
+                        H[v][c] += (double)(sum * __builtin_amdgcn_rcpf(wa * r.x));
+                    }
+                }
             }
         }
     }
     // row k = edges k and k+1: the next lane's sum, times the ray's length factor
     const float vc = rows[nv + 1 + kk];
     const float dnu = rows[kk + 1] - rows[kk];
-    float res[kFpCols];
 #pragma unroll
-    for (int c = 0; c < kFpCols; ++c) {
-        const double Hn = __shfl_down(H[c], 1);
-        const int jj = min(j0 + c, nu - 1);
-        const float sec = T[4 * nu + 2 + jj], inv = T[5 * nu + 2 + jj];
-        const float s2 = vc * inv;
-        res[c] = (float)(Hn - H[c]) * (sec * sqrtf(1.0f + s2 * s2) / dnu);
-    }
-    if (!active) return;
-    if (vec_store) {
-        *reinterpret_cast<float4*>(out_row) = make_float4(res[0], res[1], res[2], res[3]);
-    } else {
+    for (int v = 0; v < kFpViews; ++v) {
+        if (!use[v]) continue;
+        float res[kFpCols];
 #pragma unroll
-        for (int c = 0; c < kFpCols; ++c)
-            if (j0 + c < nu) out_row[c] = res[c];
+        for (int c = 0; c < kFpCols; ++c) {
+            const double Hn = __shfl_down(H[v][c], 1);
+            const int jj = min(j0 + c, nu - 1);
+            const float sec = T[v][4 * nu + 2 + jj], inv = T[v][5 * nu + 2 + jj];
+            const float s2 = vc * inv;
+            res[c] = (float)(Hn - H[v][c]) * (sec * sqrtf(1.0f + s2 * s2) / dnu);
+        }
+        if (!active) continue;
+        float* __restrict__ out_row = proj + ((size_t)(view0 + v) * nv + kk) * nu + j0;
+        if (vec_store) {
+            *reinterpret_cast<float4*>(out_row) = make_float4(res[0], res[1], res[2], res[3]);
+        } else {
+#pragma unroll
+            for (int c = 0; c < kFpCols; ++c)
+                if (j0 + c < nu) out_row[c] = res[c];
+        }
     }
 }
 
@@ -1932,7 +2000,8 @@ int ctr_dd_forward(const ctr_geometry* g, const double* angles, const int32_t* b
     if ((st = status_from_hip(hipGetLastError()))) return st;
     const int col_groups = (g->nu + kFpCols * kFpWaves - 1) / (kFpCols * kFpWaves);
     const int row_blocks = (g->nv + kColRows - 1) / kColRows;
-    const int n_vblocks = (g->n_views + kViewBlock - 1) / kViewBlock;
+    const int n_pairs = (g->n_views + kFpViews - 1) / kFpViews;
+    const int n_vblocks = (n_pairs + kViewBlock - 1) / kViewBlock;
     const long long n_blocks =
         8LL * kXcdRun * ((col_groups + 8 * kXcdRun - 1) / (8 * kXcdRun)) * row_blocks * n_vblocks * kViewBlock;
     if (n_blocks >= 2147483647LL) return CTR_ERR_INVALID;
