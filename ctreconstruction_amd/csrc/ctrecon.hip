@@ -4,6 +4,7 @@
 //   fp_geometry_kernel          per-view DD footprint tables, FP64 -> FP32 (index space)
 //   zprefix_kernel              vol[z][y][x] -> z-prefix tables [y][x][k] and [x][y][k]
 //   dd_forward_kernel           one wavefront per (2 views, 4 detector columns, 63 rows)
+//   dd_forward_zslab_kernel     z-slab views (non-circular geometries), FP64 loops per cell
 //   bp_geometry_kernel          per-view backprojection constants, FP64 -> FP32
 //   dd_backproject_tile_kernel  one workgroup per 16 x 16 x ZC voxel tile, windows in LDS
 //   adj_geometry_kernel         per-view constants of the inverse central projection
@@ -140,7 +141,7 @@ __global__ void fp_geometry_kernel(ctr_geometry g, const double* __restrict__ an
     const double Sz = g.source_init[2] + g.helical * th / (2.0 * kPi);
     const int br = branch[view];
     float* T = tabs + (size_t)view * stride;
-    if (br != CTR_BRANCH_X && br != CTR_BRANCH_Y) {  // z-slabs: rejected on the host
+    if (br != CTR_BRANCH_X && br != CTR_BRANCH_Y) {  // z-slabs: dd_forward_zslab_kernel
         if (e < nu) { T[2 * nu + 2 + e] = 0.f; }
         return;
     }
@@ -296,6 +297,9 @@ constexpr int kViewBlock = CTR_FP_VIEW_BLOCK;
 #define CTR_FP_XCD_RUN 1
 #endif
 constexpr int kXcdRun = CTR_FP_XCD_RUN;  // adjacent column groups per XCD turn
+#ifndef CTR_FP_ORDER
+#define CTR_FP_ORDER 0
+#endif
 
 // two consecutive breakpoints of a z-prefix line
 struct PTaps {
@@ -339,12 +343,22 @@ __global__ void __launch_bounds__(64 * kFpWaves, CTR_FP_MIN_WAVES) dd_forward_ke
     // of the same detector rows at the same time
     const int m = l % kXcdRun;
     const int l2 = l / kXcdRun;
+#if CTR_FP_ORDER == 1
+    const int row_blocks = (nv + kColRows - 1) / kColRows;
+    const int rb = l2 % row_blocks;
+    const int r1 = l2 / row_blocks;
+    const int vi = r1 % kViewBlock;
+    const int r3 = r1 / kViewBlock;
+    const int cg = ((r3 % runs) * 8 + xcd) * kXcdRun + m;
+    const int vp = (r3 / runs) * kViewBlock + vi;  // view pair
+#else
     const int vi = l2 % kViewBlock;
     const int r1 = l2 / kViewBlock;
     const int cg = ((r1 % runs) * 8 + xcd) * kXcdRun + m;
     const int r2 = r1 / runs;
     const int vp = (r2 % n_vblocks) * kViewBlock + vi;  // view pair
     const int rb = r2 / n_vblocks;
+#endif
     const int view0 = kFpViews * vp;
     if (view0 >= n_views || cg >= col_groups) return;
     const int lane = threadIdx.x & 63;
@@ -715,6 +729,92 @@ __global__ void __launch_bounds__(64 * kFpWaves, CTR_FP_MIN_WAVES) dd_forward_ke
             for (int c = 0; c < kFpCols; ++c)
                 if (j0 + c < nu) out_row[c] = res[c];
         }
+    }
+}
+
+// z-slab views (|S_z - D_z| dominates the source-detector offset; no circular or helical orbit
+// reaches it): kernel semantics of distance_project_on_z3 (python/GPUFuncs_pycuda.py:526-587)
+// with the host tables of distance_forward (python/Reconstruction_pycuda.py:828-864, 993-1003).
+// The slabs are z planes; at each slab the cell's four boundary lines give a footprint
+// [x1, x2] x [y1, y2], and every (x, y) cell it covers adds voxel * w_x * w_y, w the reference's
+// overlap weights (x planes increasing, y planes decreasing; weights below zero_tol dropped),
+// the sum times the cell's intersection length L.  A rare branch, so plain FP64 loops: a few
+// workgroups per view stride over its cells and return at once for x- and y-slab views.
+constexpr int kZslabGroups = 16;  // workgroups per view
+
+__global__ void __launch_bounds__(256) dd_forward_zslab_kernel(ctr_geometry g, const double* __restrict__ angles,
+                                                               const int32_t* __restrict__ branch,
+                                                               const float* __restrict__ vol,
+                                                               float* __restrict__ proj, double zero_tol) {
+    const int view = blockIdx.y;
+    if (branch[view] != CTR_BRANCH_Z) return;
+    const int nu = g.nu, nv = g.nv, nx = g.nx, ny = g.ny, nz = g.nz;
+    const double th = angles[view];
+    double s, c;
+    sincos(th, &s, &c);
+    const double Sx = -g.sad * s, Sy = g.sad * c;
+    const double Sz = g.source_init[2] + g.helical * th / (2.0 * kPi);
+    const bool flat = g.detector_shape == CTR_DETECTOR_FLAT;
+    const double step = flat ? g.du : g.du / g.sdd;
+    const double half = flat ? 0.5 * g.dv : -0.5 * g.dv;  // row edges: +dv/2 flat, -dv/2 curved
+    // edge planes (python/Reconstruction_pycuda.py:792-797): x increasing, y and z decreasing
+    const double ddx = g.dx, ddy = -g.dy, ddz = -g.dz;
+    auto xplane = [&](int i) { return g.recon_center[0] + ((double)i - (nx - 1) / 2.0) * ddx - ddx / 2.0; };
+    auto yplane = [&](int i) { return g.recon_center[1] + ((double)i - (ny - 1) / 2.0) * ddy - ddy / 2.0; };
+    auto zplane = [&](int i) { return g.recon_center[2] + ((double)i - (nz - 1) / 2.0) * ddz - ddz / 2.0; };
+    const double x0 = xplane(0), y0 = yplane(0);
+    // one DD weight (python/GPUFuncs_pycuda.py:553-575): k of [ks, ke] against the footprint
+    // [lo, hi] of width w; `dec`: decreasing planes p(k) > p(k+1)
+    auto weight = [&](int kk, int ks, int ke, double lo, double hi, double w, double pk, double pk1, double d,
+                      bool dec) {
+        double r;
+        if (ks == ke) r = 1.0;
+        else if (kk == ks) r = dec ? (hi - pk1) / w : (pk1 - lo) / w;
+        else if (kk == ke) r = dec ? (pk - lo) / w : (hi - pk) / w;
+        else r = fabs(d) / w;
+        return fabs(r) < zero_tol ? 0.0 : r;
+    };
+    for (int cell = blockIdx.x * 256 + threadIdx.x; cell < nu * nv; cell += kZslabGroups * 256) {
+        const int k = cell / nu, j = cell - k * nu;
+        // column edges j, j+1 and centre (in-plane), row edges k, k+1 and centre (z)
+        const double ue1 = ((double)j - (nu - 1) / 2.0) * step - step / 2.0 + g.det_offset[0];
+        const double ue2 = ((double)(j + 1) - (nu - 1) / 2.0) * step - step / 2.0 + g.det_offset[0];
+        const double uc = ((double)j - (nu - 1) / 2.0) * step + g.det_offset[0];
+        double bx1, by1, bx2, by2, cx, cy;
+        lattice_xy(g, c, s, Sx, Sy, ue1, &bx1, &by1);
+        lattice_xy(g, c, s, Sx, Sy, ue2, &bx2, &by2);
+        lattice_xy(g, c, s, Sx, Sy, uc, &cx, &cy);
+        const double cz = Sz + (-((double)k - (nv - 1) / 2.0) * g.dv + g.det_offset[1]);
+        const double bz1 = Sz + (-((double)k - (nv - 1) / 2.0) * g.dv + half + g.det_offset[1]);
+        const double bz2 = Sz + (-((double)(k + 1) - (nv - 1) / 2.0) * g.dv + half + g.det_offset[1]);
+        // footprint lines t = a z + b through the source: x from the column edges at the row
+        // centre, y from the column centre at the row edges
+        const double aU1 = (Sx - bx1) / (Sz - cz), bU1 = -aU1 * Sz + Sx;
+        const double aU2 = (Sx - bx2) / (Sz - cz), bU2 = -aU2 * Sz + Sx;
+        const double aV1 = (Sy - cy) / (Sz - bz2), bV1 = -aV1 * Sz + Sy;
+        const double aV2 = (Sy - cy) / (Sz - bz1), bV2 = -aV2 * Sz + Sy;
+        const double s1 = (Sx - cx) / (Sz - cz), s2 = (Sy - cy) / (Sz - cz);
+        const double L = fabs(ddz) * sqrt(1.0 + s1 * s1) * sqrt(1.0 + s2 * s2);
+        double acc = 0.0;
+        for (int n = 0; n < nz; ++n) {
+            const double zc = zplane(n) + ddz / 2.0;
+            const double t1 = aU1 * zc + bU1, t2 = aU2 * zc + bU2;
+            const double v1 = aV1 * zc + bV1, v2 = aV2 * zc + bV2;
+            const int ix1 = (int)floor((t1 - x0) / ddx), ix2 = (int)floor((t2 - x0) / ddx);
+            const int iy1 = (int)floor((v1 - y0) / ddy), iy2 = (int)floor((v2 - y0) / ddy);
+            const int sx = min(ix1, ix2), ex = max(ix1, ix2), sy = min(iy1, iy2), ey = max(iy1, iy2);
+            const double tlo = fmin(t1, t2), thi = fmax(t1, t2), wt = fabs(t1 - t2);
+            const double vlo = fmin(v1, v2), vhi = fmax(v1, v2), wv = fabs(v1 - v2);
+            const float* slab = vol + (size_t)n * nx * ny;
+            for (int kx = max(sx, 0); kx <= min(ex, nx - 1); ++kx) {
+                const double w1 = weight(kx, sx, ex, tlo, thi, wt, xplane(kx), xplane(kx + 1), ddx, false);
+                for (int ly = max(sy, 0); ly <= min(ey, ny - 1); ++ly) {
+                    const double w2 = weight(ly, sy, ey, vlo, vhi, wv, yplane(ly), yplane(ly + 1), ddy, true);
+                    acc += (double)slab[(size_t)ly * nx + kx] * w1 * w2;
+                }
+            }
+        }
+        proj[((size_t)view * nv + k) * nu + j] = (float)(acc * L);
     }
 }
 
@@ -1938,7 +2038,6 @@ int ctr_select_branches(const ctr_geometry* g, const double* angles, int32_t* br
     int st = check_geometry(g);
     if (st) return st;
     if (!angles || !branch_out) return CTR_ERR_INVALID;
-    int rc = CTR_OK;
     for (int i = 0; i < g->n_views; ++i) {
         const double th = angles[i];
         const double z = g->helical * th / (2.0 * kPi);
@@ -1953,10 +2052,9 @@ int ctr_select_branches(const ctr_geometry* g, const double* angles, int32_t* br
             branch_out[i] = CTR_BRANCH_Y;
         } else {
             branch_out[i] = CTR_BRANCH_Z;
-            rc = CTR_ERR_UNSUPPORTED;
         }
     }
-    return rc;
+    return CTR_OK;
 }
 
 int ctr_zero_padded_length(int nu) {
@@ -2008,6 +2106,10 @@ int ctr_dd_forward(const ctr_geometry* g, const double* angles, const int32_t* b
     hipLaunchKernelGGL(dd_forward_kernel, dim3((unsigned)n_blocks), dim3(64 * kFpWaves), 0, s, t_ys, t_xs,
                        proj, tabs, rows, branch, g->nx, g->ny, g->nz, nzp, g->nu, g->nv, stride, 1e-6f,
                        g->n_views, col_groups, n_vblocks);
+    if ((st = status_from_hip(hipGetLastError()))) return st;
+    // z-slab views (left at zero above): their own kernel, which returns at once for the others
+    hipLaunchKernelGGL(dd_forward_zslab_kernel, dim3(kZslabGroups, g->n_views), dim3(256), 0, s, *g, angles,
+                       branch, vol, proj, 1e-6);
     return status_from_hip(hipGetLastError());
 }
 

@@ -144,5 +144,8 @@ class ScanGeometry:
         return sub
 
     def check_supported(self):
+        """The matched adjoint (and so autograd, SART, ASD-POCS) covers x- and y-slab views only;
+        the forward projector and the FDK backprojector take every view."""
         if np.any(self.branch == N.CTR_BRANCH_Z):
-            raise N.CtrError(N.CTR_ERR_UNSUPPORTED, "z-slab branch (|S_z - D_z| dominates)")
+            raise N.CtrError(N.CTR_ERR_UNSUPPORTED,
+                             "matched adjoint of z-slab views (|S_z - D_z| dominates) is not built")

@@ -46,7 +46,6 @@ def dd_forward(vol: torch.Tensor, geom: ScanGeometry, out: torch.Tensor | None =
     Replaces Reconstruction.forward/distance_forward (python/Reconstruction_pycuda.py:750-1059).
     """
     _require(vol, geom.vol_shape, "vol")
-    geom.check_supported()
     dev = vol.device
     if out is None:
         out = torch.empty(geom.proj_shape, dtype=torch.float32, device=dev)

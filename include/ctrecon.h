@@ -29,7 +29,7 @@ extern "C" {
 enum ctr_status {
     CTR_OK = 0,
     CTR_ERR_INVALID = 1,      /* bad geometry, null pointer, size out of range            */
-    CTR_ERR_UNSUPPORTED = 2,  /* z-slab branch, unknown detector shape, FFT length > 8192 */
+    CTR_ERR_UNSUPPORTED = 2,  /* unknown detector shape, FFT length > 8192, sizes past 31-bit offsets */
     CTR_ERR_WORKSPACE = 3,    /* workspace smaller than the *_workspace_bytes() query     */
     CTR_ERR_HIP = 4,          /* a HIP launch or runtime call failed                       */
     CTR_ERR_SCRATCH = 5       /* too large for ctr_dd_backproject_static's static scratch  */
@@ -78,9 +78,8 @@ int ctr_version(void);
 const char* ctr_strerror(int code);
 
 /* Host helper: slab branch of every view from the reference's FP64 comparisons
- * (python/Reconstruction_pycuda.py:828-830, 852-853, 921-922; ties go to x, then y).
- * `angles` and `branch_out` are HOST arrays of n_views entries.  Returns
- * CTR_ERR_UNSUPPORTED if a view selects the z-slab branch. */
+ * (python/Reconstruction_pycuda.py:828-830, 852-853, 921-922; ties go to x, then y, then z).
+ * `angles` and `branch_out` are HOST arrays of n_views entries. */
 int ctr_select_branches(const ctr_geometry* g, const double* angles, int32_t* branch_out);
 
 /* Zero-padded FFT length 2^ceil(log2(2(nu-1))) (python/Reconstruction_pycuda.py:236). */
@@ -88,10 +87,12 @@ int ctr_zero_padded_length(int nu);
 
 /* ---- DD forward projection --------------------------------------------------------------
  * Replaces Reconstruction.distance_forward's per-view host geometry and H2D copies
- * (python/Reconstruction_pycuda.py:762-1059) and the kernels distance_project_on_{x,y}3
- * (python/GPUFuncs_pycuda.py:398-524).  Per-view footprint tables are built on the GPU in
- * FP64; the projector is a gather (one work-item per detector cell, slabs looped in
- * registers, no atomics, deterministic).  `proj` is overwritten.  `vol` and `proj` must not
+ * (python/Reconstruction_pycuda.py:762-1059) and the kernels distance_project_on_{x,y,z}3
+ * (python/GPUFuncs_pycuda.py:398-587).  Per-view footprint tables are built on the GPU in
+ * FP64; the projector is a gather (slabs summed in registers per detector cell, no atomics,
+ * deterministic).  z-slab views (the z branch, which only non-circular source/detector offsets
+ * select) go through a plain FP64 kernel whose cost grows with the footprint area.  `proj` is
+ * overwritten.  `vol` and `proj` must not
  * overlap the workspace.  CTR_ERR_UNSUPPORTED when 256 * (nz + 2) * max(nx, ny) >= 2^31
  * (e.g. above 2048 x 2048 x 2046). */
 size_t ctr_dd_forward_workspace_bytes(const ctr_geometry* g);
@@ -128,8 +129,9 @@ int ctr_fdk_filter(const ctr_geometry* g, const double* spectrum, const float* p
  * nothing either, but runs every (voxel, view) pair in one pass instead of a common-case pass
  * followed by a completing pass (the same sums in another order: a check of that split).
  * mode CTR_BP_ADJOINT computes scale * A^T proj, the exact transpose of ctr_dd_forward (its
- * weights and its 1e-6 transverse zeroing; `zero_tol` is ignored); it needs `branch` and has no
- * counterpart in the reference, which only has the FDK-weighted backprojector. */
+ * weights and its 1e-6 transverse zeroing; `zero_tol` is ignored) over the x- and y-slab views;
+ * z-slab views add nothing.  It needs `branch` and has no counterpart in the reference, which
+ * only has the FDK-weighted backprojector. */
 size_t ctr_dd_backproject_workspace_bytes(const ctr_geometry* g, int mode);
 int ctr_dd_backproject(const ctr_geometry* g, const double* angles, const int32_t* branch,
                        const float* proj, float* vol, int mode, float scale, float zero_tol,

@@ -67,16 +67,20 @@ def test_branch_selection_matches_host_geometry(lib, n):
     assert set(np.unique(out)) <= {0, 1}
 
 
-def test_branch_selection_rejects_z_slabs(lib):
+def test_branch_selection_marks_z_slabs(lib):
     geom = ScanGeometry(make_params(SourceInit=[0, 10.0, 900.0], DetectorInit=[0, -10.0, -900.0]))
     g = geom.ctypes_struct()
     ang = np.ascontiguousarray(geom.angles)
     out = np.zeros(geom.nview, dtype=np.int32)
     rc = lib.ctr_select_branches(ctypes.byref(g), ang.ctypes.data_as(ctypes.c_void_p),
                                  out.ctypes.data_as(ctypes.c_void_p))
-    # |S_z - D_z| = 1800 beats SDD*max(|sin|,|cos|) near 45 degrees only
-    assert rc == N.CTR_ERR_UNSUPPORTED
+    # |S_z - D_z| = 1800 beats SDD*max(|sin|,|cos|) away from the axes: z-slab views, which the
+    # forward projector takes (the matched adjoint does not)
+    assert rc == N.CTR_OK
     assert out[1] == 2 and out[0] == 1
+    np.testing.assert_array_equal(out, geom.branch)
+    with pytest.raises(N.CtrError):
+        geom.check_supported()
 
 
 def test_invalid_geometry_is_rejected_before_launch(lib):

@@ -105,6 +105,27 @@ def test_backproject_matches_oracle(dev, name, mode):
     assert err <= TOL
 
 
+def test_forward_z_slab_views_match_oracle(dev):
+    """Views that select the z-slab branch (steep rays; the oracle's z branch is pinned by brute
+    force and a uniform-volume closed form) next to x- and y-slab views of the same scan."""
+    from tests.test_oracle_projectors import ZSLAB
+    p = make_params(**ZSLAB)
+    g = O.Geometry(p)
+    geom = ctr.ScanGeometry(p)
+    assert list(geom.branch) == [1, 2, 0, 2, 1, 2, 0, 2]
+    vol = _vol(g, 17)
+    out = ctr.dd_forward(torch.from_numpy(vol).to(dev), geom).cpu().numpy()
+    ref = O.forward_project(g, vol.astype(np.float64))
+    zv = geom.branch == N.CTR_BRANCH_Z
+    e_z, e_xy = _rel(out[zv], ref[zv]), _rel(out[~zv], ref[~zv])
+    print(f"forward z-slab views {e_z:.2e}, x/y-slab views {e_xy:.2e}")
+    assert np.abs(ref[zv]).max() > 0
+    assert e_z <= TOL and e_xy <= TOL
+    # the matched adjoint covers x- and y-slab views only, and says so
+    with pytest.raises(N.CtrError):
+        ctr.dd_backproject(torch.from_numpy(out).to(dev), geom, mode="adjoint")
+
+
 def test_backproject_numba_weight_zeroing(dev):
     # the numba kernels zero overlap weights below 1e-5 (python/GPUFuncs_numba.py:28-29, 45-46)
     p = make_params(**CASES["flat"])

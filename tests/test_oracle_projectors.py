@@ -40,6 +40,54 @@ def test_uniform_box_off_centre_rays_closed_form():
     np.testing.assert_allclose(pr[4:11, 6:15], cf[4:11, 6:15], rtol=1e-13)
 
 
+# z-slab views: the detector rows sit about 1500 mm below the source height at 922 mm in-plane
+# distance, so the rays are steep and every view away from the x and y axes selects the z-slab
+# branch (python/GPUFuncs_pycuda.py:526-587); the volume sits where the rays cross at SAD.
+ZSLAB = dict(NumberOfImage=[16, 16, 16], ImagePixelSpacing=[0.5, 0.5, 0.5], NumberOfDetectorPixels=[16, 16],
+             DetectorPixelSize=[5.0, 5.0], SourceInit=[0, 100.0, 0.0], DetectorInit=[0, -100.0, -900.0],
+             RotationOrigin=[0, 0, 0], DetectorOffset=[0.0, -1502.0], ReconCenter=[0.0, 0.0, -162.9],
+             NumberOfViews=8, StartAngle=0.0, EndAngle=2 * np.pi)
+
+
+def test_z_branch_selected_away_from_the_axes():
+    g = O.Geometry(make_params(**ZSLAB))
+    br = [O.select_branch(g.source_at(i), g.detector_at(i)) for i in range(g.nview)]
+    assert br == [1, 2, 0, 2, 1, 2, 0, 2]
+
+
+def test_z_branch_matches_brute_force():
+    g = O.Geometry(make_params(**dict(ZSLAB, NumberOfImage=[7, 6, 8], NumberOfDetectorPixels=[9, 7])))
+    vol = np.random.default_rng(11).random((8, 6, 7))
+    for i in (1, 3):
+        a = O.forward_project_view(g, vol, i)
+        b = np.array(B.forward_view(g, vol, i))
+        assert np.abs(a).max() > 0
+        np.testing.assert_allclose(a, b, rtol=0, atol=1e-12)
+
+
+def test_z_branch_uniform_volume_closed_form():
+    # A ray that stays inside the volume crosses every z slab; the footprint weights of one slab
+    # sum to 1 in x and in y, so p = nz * |dz| * sqrt(1+s1^2) * sqrt(1+s2^2) with s1, s2 the
+    # ray's x and y slopes against z (Q5 form of L).
+    g = O.Geometry(make_params(**ZSLAB))
+    xp, yp, zp = g.planes()
+    for i in (1, 3, 5, 7):
+        pr = O.forward_project_view(g, np.ones((16, 16, 16)), i)
+        S = g.source_at(i)
+        C, _ = O.detector_lattice(g, S, g.angles[i])
+        s1 = (S[0] - C[0]) / (S[2] - C[2])
+        s2 = (S[1] - C[1]) / (S[2] - C[2])
+        cf = 16 * 0.5 * np.sqrt(1 + s1 ** 2) * np.sqrt(1 + s2 ** 2)
+        # rays whose x and y stay a voxel inside the box from the top plane to the bottom one
+        inside = np.ones(pr.shape, dtype=bool)
+        for z in (zp[0], zp[-1]):
+            x = S[0] + s1 * (z - S[2])
+            y = S[1] + s2 * (z - S[2])
+            inside &= (x > xp.min() + 1.0) & (x < xp.max() - 1.0) & (y > yp.min() + 1.0) & (y < yp.max() - 1.0)
+        assert inside.sum() >= 4
+        np.testing.assert_allclose(pr[inside], cf[inside], rtol=1e-12)
+
+
 @pytest.mark.parametrize("shape", ["Flat", "Curved"])
 def test_forward_matches_brute_force(shape):
     g = O.Geometry(make_params(NumberOfImage=[6, 7, 5], NumberOfDetectorPixels=[7, 5],
