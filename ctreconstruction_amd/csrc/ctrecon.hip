@@ -261,8 +261,9 @@ __global__ void __launch_bounds__(256) zprefix_kernel(const float* __restrict__ 
 // the row edges e, and its overlap-weighted sum over a z line is the line's z integral
 // I(c_k+1) - I(c_k), interpolated in the z-prefix table (the two breakpoints around the edge).
 // Per slab the wavefront stages the z segment of the union of its column-views' lines in LDS
-// with one to four coalesced 16-byte-per-lane loads (lanes past the segment get an out-of-range
-// offset and fetch nothing).  Lane L owns edge e0 + L of kColRows + 1 = 64 consecutive edges and
+// with one to four coalesced 16-byte-per-lane loads that write LDS directly (buffer_load ...
+// lds: the data never passes through VGPRs, and no ds_write is issued; lanes past the segment
+// get an out-of-range offset and fetch nothing).  Lane L owns edge e0 + L of kColRows + 1 = 64 consecutive edges and
 // sums, in FP64 over the slabs and per column-view, H_e = sum_slab sum_a w_a I_a(c_e) with the
 // transverse weights w_a pre-divided by wa * tt; row k is then (H_k+1 - H_k) / (nu_k+1 - nu_k),
 // scaled by the ray's length factor, and the four columns of a row leave in one 16-byte store.
@@ -426,11 +427,12 @@ __global__ void __launch_bounds__(64 * kFpWaves, CTR_FP_MIN_WAVES) dd_forward_ke
     for (int v = 0; v < kFpViews; ++v)
 #pragma unroll
         for (int c = 0; c < kFpCols; ++c) H[v][c] = 0.0;
-    __shared__ uint4 segs[kFpWaves][kFpMaxQuads];
+    // two segment buffers per wavefront: a pair of slabs is loaded straight into LDS
+    // (buffer_load ... lds: no VGPRs, no ds_write) while nothing of the pair is read yet
+    __shared__ uint4 segs[kFpWaves][2][kFpMaxQuads];
     __shared__ float4 recs[kFpWaves][kFpViews][64];
-    uint4* __restrict__ seg = segs[wave];
-    const char* segs_bytes = reinterpret_cast<const char*>(&segs[0][0]);
-    const int seg_off = wave * kFpMaxQuads * 16;  // this wavefront's segment buffer in segs
+    const char* segs_bytes = reinterpret_cast<const char*>(&segs[0][0][0]);
+    const int seg_off = wave * 2 * kFpMaxQuads * 16;  // this wavefront's segment buffers in segs
     // lanes g = 64 i + lane of segment load i: (g + 1/2) as a float, for the line index
     float gh[kFpMaxLoads];
 #pragma unroll
@@ -558,17 +560,18 @@ __global__ void __launch_bounds__(64 * kFpWaves, CTR_FP_MIN_WAVES) dd_forward_ke
                 }
             }
             const uint64_t heads = __ballot(cl == 0);
-            // staged slabs, software-pipelined in pairs: both slabs' segment loads go out, then
-            // both are summed; no load result lives across rounds, so nothing waits to be copied
+            // staged slabs, software-pipelined in pairs: both slabs' segments are loaded into the
+            // wavefront's two LDS buffers, then both are summed; the next pair's loads go out only
+            // after every read of this pair has returned (its sums use them)
             auto walk_slabs = [&](auto nl_c, auto lines_c, uint64_t mask) {
                 constexpr int NL = decltype(nl_c)::value;        // segment loads per slab
                 constexpr int LINES = decltype(lines_c)::value;  // lines read per column
                 struct Set {
-                    v4u d[NL];
-                    int q;
+                    int q, b;  // slab, LDS buffer
                 };
-                auto issue = [&](int q, Set& S) {
+                auto issue = [&](int q, int b, Set& S) {
                     S.q = q;
+                    S.b = b;
                     const int U1 = __builtin_amdgcn_readlane(u1, 4 * q), U2 = __builtin_amdgcn_readlane(u2, 4 * q);
                     const int L0 = U1 & 0xffffff, nlines = (unsigned)U1 >> 24;
                     const int kseg = U2 & 0xffff, nq = (U2 >> 16) & 0xff;
@@ -585,13 +588,13 @@ __global__ void __launch_bounds__(64 * kFpWaves, CTR_FP_MIN_WAVES) dd_forward_ke
                         // quads past the segment get an offset beyond the buffer: the range
                         // check returns 0 for them without a memory access
                         const int voff = ln < nlines ? 16 * (64 * i + lane) + __mul24(ln, step) : 0x7ffffff0;
-                        S.d[i] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, voff, soff, 0);
+                        // lane L's 16 bytes land at quad 64 i + L of the buffer
+                        __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                            rsrc, (__attribute__((address_space(3))) void*)&segs[wave][b][64 * i], 16, voff, soff, 0, 0);
                     }
                 };
                 auto consume = [&](const Set& S) {
-#pragma unroll
-                    for (int i = 0; i < NL; ++i)
-                        seg[64 * i + lane] = make_uint4(S.d[i].x, S.d[i].y, S.d[i].z, S.d[i].w);
+                    const int buf_off = seg_off + S.b * kFpMaxQuads * 16;
                     const int U2 = __builtin_amdgcn_readlane(u2, 4 * S.q);
                     const int kseg = U2 & 0xffff, nq16 = (U2 >> 12) & 0xff0;
 #pragma unroll
@@ -608,13 +611,13 @@ __global__ void __launch_bounds__(64 * kFpWaves, CTR_FP_MIN_WAVES) dd_forward_ke
                             const int pc = __builtin_amdgcn_readlane(p[v], 4 * S.q + c);
                             // byte of the column's first line in the segment buffers, less the
                             // segment's first breakpoint (scalar): one shift-add per lane
-                            const unsigned sbase = (unsigned)(seg_off + (pc & 0xff) * nq16 - 4 * kseg);
+                            const unsigned sbase = (unsigned)(buf_off + (pc & 0xff) * nq16 - 4 * kseg);
                             const float cz = __builtin_amdgcn_fmed3f(fmaf(nue, r[c].x, gam_v[v]), 0.0f, fnz);
                             const float fl = floorf(cz);
                             fr[c] = cz - fl;
                             // a column that misses this slab reads the segment's start with
                             // weight 0
-                            lb[c] = (pc & kRecHit) ? 4u * (unsigned)fl + sbase : (unsigned)seg_off;
+                            lb[c] = (pc & kRecHit) ? 4u * (unsigned)fl + sbase : (unsigned)buf_off;
                         }
                         PTaps t[kFpCols][LINES];
 #pragma unroll
@@ -634,17 +637,17 @@ __global__ void __launch_bounds__(64 * kFpWaves, CTR_FP_MIN_WAVES) dd_forward_ke
                     }
                 };
                 // the rare walks (three or four segment loads, or three lines per column) go one
-                // slab at a time, which keeps their registers within the common walks' budget
+                // slab at a time (pairing them measured slower)
                 constexpr bool kPair = NL <= 2 && LINES == 2;
                 while (mask) {
                     Set S0, S1;
                     const int q0 = __builtin_ctzll(mask) >> 2;
                     mask &= mask - 1;
-                    issue(q0, S0);
+                    issue(q0, 0, S0);
                     if (kPair && mask) {
                         const int q1 = __builtin_ctzll(mask) >> 2;
                         mask &= mask - 1;
-                        issue(q1, S1);
+                        issue(q1, 1, S1);
                         __builtin_amdgcn_sched_barrier(0);
                         consume(S0);
                         consume(S1);
