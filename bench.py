@@ -454,16 +454,21 @@ def main(argv=None):
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "algorithmic_bytes": bp_bytes, "traffic": bp_prof.get("traffic_bytes"),
                      "texture_busy": {"ta": bp_prof.get("ta_busy_frac"), "td": bp_prof.get("td_busy_frac")},
-                     # what bounds it: VALU issue (a wave64 FP32 instruction holds its SIMD for
-                     # 4 cycles, so 0.25 wave-instructions per SIMD-cycle is full rate)
+                     # what bounds it: instruction issue (the measured lone-instruction ceilings of
+                     # --calibrate-valu: v_fma_f32 0.28, v_add_u32 0.43 wave-instructions per
+                     # SIMD-cycle), with the LDS busy beside it; not HBM
+                     "limiter": "instruction issue (VALU), LDS beside it",
                      "valu_issue_per_simd_cycle": bp_prof.get("valu_issue_per_simd_cycle"),
-                     "valu_lane_instructions_per_update": bp_prof.get("valu_lane_instructions_per_update")},
+                     "valu_lane_instructions_per_update": bp_prof.get("valu_lane_instructions_per_update"),
+                     "lds_active_frac": bp_prof.get("lds_active_frac")},
         "fdk_backprojection": {"s": bp_s, "gups": updates / bp_s / 1e9},
         "fdk_filter": {"s": sec - bp_s},
         "dd_forward": {"s": fp_s, "gups": updates / fp_s / 1e9, "ray_slab_updates_per_s": ray_slab / fp_s,
                        "hbm_traffic_bytes": fp_prof.get("traffic_bytes"),
                        "texture_busy": {"ta": fp_prof.get("ta_busy_frac"), "td": fp_prof.get("td_busy_frac")},
-                       "valu_issue_per_simd_cycle": fp_prof.get("valu_issue_per_simd_cycle")},
+                       "valu_issue_per_simd_cycle": fp_prof.get("valu_issue_per_simd_cycle"),
+                       "limiter": "LDS (staged z segments and their breakpoint reads)",
+                       "lds_active_frac": fp_prof.get("lds_active_frac")},
         "sart_vs_learned_time_to_rmse": None if args.no_learned else learned_vs_sart(),
         "pixel_fdk": None if args.no_pixel else pixel_fdk_leg(dev, stream, args.steps, args.warmup,
                                                              cpu=not args.no_cpu_baseline),
