@@ -276,6 +276,7 @@ __global__ void __launch_bounds__(256) zprefix_kernel(const float* __restrict__ 
 // explicit per-line path that reads the breakpoint pairs straight from memory.
 constexpr int kFpCols = 4;              // detector columns per wavefront
 constexpr int kFpViews = 2;             // adjacent views per wavefront
+constexpr float kFpPairShift = 8.0f;    // most a column's lines may move between the views of a walk
 constexpr int kFpBatch = 64 / kFpCols;  // slabs per batch (one lane per slab and column)
 constexpr int kColRows = 63;            // detector rows per wavefront (64 lanes = its 64 row edges)
 #ifndef CTR_FP_WAVES
@@ -388,15 +389,6 @@ __global__ void __launch_bounds__(64 * kFpWaves, CTR_FP_MIN_WAVES) dd_forward_ke
         }
     }
     if (!use[0] && !use[1]) return;
-    // walks over the slabs: both views at once when they share the slab orientation, else one
-    // view per walk (view mask bit v)
-    int walks[kFpViews], n_walks = 0;
-    if (use[0] && use[1] && br[0] == br[1]) {
-        walks[n_walks++] = 3;
-    } else {
-        if (use[0]) walks[n_walks++] = 1;
-        if (use[1]) walks[n_walks++] = 2;
-    }
     const float fnz = (float)nz;
     // z extent of the wavefront's edges
     const float nue = rows[min(k, nv)];  // this lane's edge slope
@@ -421,6 +413,26 @@ __global__ void __launch_bounds__(64 * kFpWaves, CTR_FP_MIN_WAVES) dd_forward_ke
         b_be[v] = T[v][nu + 2 + jc];
         T0[v] = T[v][2 * nu + 2 + jc];
         dT[v] = T[v][3 * nu + 2 + jc];
+    }
+    // walks over the slabs: both views at once when they share the slab orientation and their
+    // rays cross nearly the same lines (the footprint lines of each column move by at most
+    // kFpPairShift cells between the two views at the first and the last slab: a fine angular
+    // step), else one view per walk (view mask bit v); a coarse step would make the union of
+    // lines wide and the shared segment slower than two narrow ones
+    bool together = use[0] && use[1] && br[0] == br[1];
+    if (together) {
+        const float fN = (float)((br[0] == CTR_BRANCH_X ? nx : ny) - 1);
+        float d = fmaxf(fabsf(a_al[0] - a_al[1]), fabsf(b_al[0] - b_al[1]));
+        d = fmaxf(d, fabsf(fmaf(a_be[0], fN, a_al[0]) - fmaf(a_be[1], fN, a_al[1])));
+        d = fmaxf(d, fabsf(fmaf(b_be[0], fN, b_al[0]) - fmaf(b_be[1], fN, b_al[1])));
+        together = !__any(col_ok && !(d <= kFpPairShift));
+    }
+    int walks[kFpViews], n_walks = 0;
+    if (together) {
+        walks[n_walks++] = 3;
+    } else {
+        if (use[0]) walks[n_walks++] = 1;
+        if (use[1]) walks[n_walks++] = 2;
     }
     double H[kFpViews][kFpCols];
 #pragma unroll

@@ -28,6 +28,10 @@ CASES = {
                         StartAngle=-0.2, EndAngle=np.pi),
     "helical": dict(NumberOfImage=[20, 20, 24], NumberOfDetectorPixels=[36, 20], NumberOfViews=16,
                     Pitch=0.5, EndAngle=4 * np.pi),
+    # a fine angular step: the forward projector walks each pair of adjacent views together
+    # (one staged segment for both), including pairs that straddle a change of slab orientation
+    "fine_steps": dict(NumberOfImage=[24, 24, 20], NumberOfDetectorPixels=[40, 32], NumberOfViews=90,
+                       StartAngle=0.01, EndAngle=0.01 + 2 * np.pi),
     "curved": dict(NumberOfImage=[24, 24, 16], NumberOfDetectorPixels=[48, 30], NumberOfViews=12,
                    DetectorShape="Curved"),
     # 2 mm voxels seen by 0.2 mm pixels: more u taps than are held in registers, detector windows
