@@ -238,7 +238,7 @@ __global__ void __launch_bounds__(256) zprefix_kernel(const float* __restrict__ 
     }
 }
 
-// DD forward projection.  Kernel semantics of distance_project_on_{y,x}3
+// DD forward projection of x- and y-slab views.  Kernel semantics of distance_project_on_{y,x}3
 // (python/GPUFuncs_pycuda.py:398-524): at every slab the cell's four boundary lines give a
 // footprint [a1,a2] x [c1,c2] that is overlapped with the voxel grid; the reference's
 // per-(slab, cell) atomicAdd becomes a register sum over slabs and one store.  Coordinates
@@ -251,11 +251,12 @@ __global__ void __launch_bounds__(256) zprefix_kernel(const float* __restrict__ 
 // the wavefront.  In a batch of kFpBatch slabs, lane 4 s + c works them out for slab n0 + s,
 // column c and both views; the four lanes of slab s agree (quad swizzles) on the union of the
 // eight column-views' lines and z breakpoints, and the wavefront then walks the batch slab by
-// slab.  Adjacent views cross nearly the same lines (their rays turn by one angular step, which
-// moves a ray's crossing with a slab by a cell or two), so one staged segment serves both: the
-// prefix data fetched from beyond L2 per view falls by about a third.  The two views of a pair
-// are projected together when they use the same slab orientation; a pair that straddles a
-// change of orientation walks the slabs once per view.
+// slab.  At a fine angular step adjacent views cross nearly the same lines (their rays turn by
+// one step, which moves a ray's crossing with a slab by a cell or a few), so one staged segment
+// serves both: the prefix data fetched from beyond L2 per view falls by about a fifth.  The two
+// views of a pair are projected together when they use the same slab orientation and every
+// column's lines move by at most kFpPairShift cells between them; otherwise (a change of
+// orientation, a coarse step) the wavefront walks the slabs once per view.
 //
 // z direction: the footprint of row k at a slab is [c_k, c_k+1] with c_e = nu_e tt + gamma on
 // the row edges e, and its overlap-weighted sum over a z line is the line's z integral
@@ -263,9 +264,9 @@ __global__ void __launch_bounds__(256) zprefix_kernel(const float* __restrict__ 
 // Per slab the wavefront stages the z segment of the union of its column-views' lines in LDS
 // with one to four coalesced 16-byte-per-lane loads that write LDS directly (buffer_load ...
 // lds: the data never passes through VGPRs, and no ds_write is issued; lanes past the segment
-// get an out-of-range offset and fetch nothing).  Lane L owns edge e0 + L of kColRows + 1 = 64 consecutive edges and
-// sums, in FP64 over the slabs and per column-view, H_e = sum_slab sum_a w_a I_a(c_e) with the
-// transverse weights w_a pre-divided by wa * tt; row k is then (H_k+1 - H_k) / (nu_k+1 - nu_k),
+// get an out-of-range offset and fetch nothing).  Lane L owns edge e0 + L of kColRows + 1 = 64
+// consecutive edges and sums, in FP64 over the slabs and per column-view,
+// H_e = sum_slab sum_a w_a I_a(c_e) with the transverse weights w_a pre-divided by wa * tt; row k is then (H_k+1 - H_k) / (nu_k+1 - nu_k),
 // scaled by the ray's length factor, and the four columns of a row leave in one 16-byte store.
 // The width |c_k+1 - c_k| = |nu_k+1 - nu_k| |tt| is the reference's normaliser; a footprint
 // inside one cell gets weight 1 (up to rounding) as in the reference, and cells outside the
