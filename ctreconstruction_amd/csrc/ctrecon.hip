@@ -368,12 +368,13 @@ __global__ void __launch_bounds__(64 * kFpWaves, CTR_FP_MIN_WAVES) dd_forward_ke
     // wave-uniform: lets the compiler keep column data and addresses in scalar registers
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     const int j0 = (cg * kFpWaves + wave) * kFpCols;  // first column of this wavefront
-    if (j0 >= nu) return;
+    // a wavefront whose columns are all past the detector has no work but stays for the
+    // workgroup's final barrier
+    const bool live = j0 < nu;
     const int e0 = rb * kColRows;
     const int k = e0 + lane;  // this lane's row (lanes 0 .. kColRows-1) and edge
-    const bool active = lane < kColRows && k < nv;
+    const bool active = live && lane < kColRows && k < nv;
     const int kk = min(k, nv - 1);
-    const bool vec_store = (nu & 3) == 0 && j0 + kFpCols <= nu && (reinterpret_cast<uintptr_t>(proj) & 15) == 0;
     // the pair's views: present, slab orientation (x or y slabs; z slabs write zeros)
     int br[kFpViews];
     bool use[kFpViews];
@@ -429,9 +430,9 @@ __global__ void __launch_bounds__(64 * kFpWaves, CTR_FP_MIN_WAVES) dd_forward_ke
         together = !__any(col_ok && !(d <= kFpPairShift));
     }
     int walks[kFpViews], n_walks = 0;
-    if (together) {
+    if (live && together) {
         walks[n_walks++] = 3;
-    } else {
+    } else if (live) {
         if (use[0]) walks[n_walks++] = 1;
         if (use[1]) walks[n_walks++] = 2;
     }
@@ -721,9 +722,14 @@ __global__ void __launch_bounds__(64 * kFpWaves, CTR_FP_MIN_WAVES) dd_forward_ke
             }
         }
     }
-    // row k = edges k and k+1: the next lane's sum, times the ray's length factor
+    // row k = edges k and k+1: the next lane's sum, times the ray's length factor.  The rows
+    // leave through LDS: each wavefront parks its four columns per row in its own segment buffer
+    // ([view][row][column]), and after one barrier the workgroup writes each row's 16 columns as
+    // one 64-byte run, so that no cache line is written back in pieces by wavefronts that finish
+    // at different times.
     const float vc = rows[nv + 1 + kk];
     const float dnu = rows[kk + 1] - rows[kk];
+    float4* __restrict__ stage = reinterpret_cast<float4*>(&segs[wave][0][0]);
 #pragma unroll
     for (int v = 0; v < kFpViews; ++v) {
         if (!use[v]) continue;
@@ -736,14 +742,28 @@ __global__ void __launch_bounds__(64 * kFpWaves, CTR_FP_MIN_WAVES) dd_forward_ke
             const float s2 = vc * inv;
             res[c] = (float)(Hn - H[v][c]) * (sec * sqrtf(1.0f + s2 * s2) / dnu);
         }
-        if (!active) continue;
-        float* __restrict__ out_row = proj + ((size_t)(view0 + v) * nv + kk) * nu + j0;
-        if (vec_store) {
-            *reinterpret_cast<float4*>(out_row) = make_float4(res[0], res[1], res[2], res[3]);
+        stage[v * 64 + lane] = make_float4(res[0], res[1], res[2], res[3]);
+    }
+    __syncthreads();
+    // item (v, r, q): view v, row e0 + r, the 4 columns of wavefront q; consecutive threads take
+    // the wavefronts of one row
+    const int jw = cg * kFpWaves * kFpCols;  // the workgroup's first column
+    const bool vec_ok = (nu & 3) == 0 && (reinterpret_cast<uintptr_t>(proj) & 15) == 0;
+    for (int it = threadIdx.x; it < kFpViews * kColRows * kFpWaves; it += 64 * kFpWaves) {
+        const int q = it % kFpWaves;
+        const int r = (it / kFpWaves) % kColRows;
+        const int v = it / (kFpWaves * kColRows);
+        const int row = e0 + r, jq = jw + q * kFpCols;
+        if (!use[v] || row >= nv || jq >= nu) continue;
+        const float4 val = reinterpret_cast<const float4*>(&segs[q][0][0])[v * 64 + r];
+        float* __restrict__ o = proj + ((size_t)(view0 + v) * nv + row) * nu + jq;
+        if (vec_ok && jq + kFpCols <= nu) {
+            *reinterpret_cast<float4*>(o) = val;
         } else {
+            const float vv[kFpCols] = {val.x, val.y, val.z, val.w};
 #pragma unroll
             for (int c = 0; c < kFpCols; ++c)
-                if (j0 + c < nu) out_row[c] = res[c];
+                if (jq + c < nu) o[c] = vv[c];
         }
     }
 }
