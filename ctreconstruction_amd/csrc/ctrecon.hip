@@ -267,7 +267,8 @@ __global__ void __launch_bounds__(256) zprefix_kernel(const float* __restrict__ 
 // get an out-of-range offset and fetch nothing).  Lane L owns edge e0 + L of kColRows + 1 = 64
 // consecutive edges and sums, in FP64 over the slabs and per column-view,
 // H_e = sum_slab sum_a w_a I_a(c_e) with the transverse weights w_a pre-divided by wa * tt; row k is then (H_k+1 - H_k) / (nu_k+1 - nu_k),
-// scaled by the ray's length factor, and the four columns of a row leave in one 16-byte store.
+// scaled by the ray's length factor.  Finished rows go through the wavefront's LDS buffer so that
+// the workgroup writes each detector row's 16 columns as one 64-byte run.
 // The width |c_k+1 - c_k| = |nu_k+1 - nu_k| |tt| is the reference's normaliser; a footprint
 // inside one cell gets weight 1 (up to rounding) as in the reference, and cells outside the
 // volume are cut by clamping c_e to [0, nz].  Transverse overlaps below zero_tol times the
