@@ -11,7 +11,7 @@
 //   adj_rowprefix_kernel        row prefix sums of projections times path length
 //   dd_adjoint_kernel           exact transpose of the forward projector, gathered per voxel
 //   spectrum_kernel             filter magnitude from the Ram-Lak cosine series, FP64
-//   twiddle_kernel              exp(-2 pi i k / N) table, FP64 -> FP32
+//   twiddle_kernel              Stockham twiddles, stage-major, FP64 -> FP32
 //   fdk_filter_kernel           cosine weight + LDS Stockham FFT convolution, two rows per FFT
 //
 // Reference semantics are cited at each kernel (leehoy/CTReconstruction paths).
@@ -1804,12 +1804,24 @@ __global__ void __launch_bounds__(256) spectrum_kernel(double* __restrict__ out,
     out[f] = H;
 }
 
-__global__ void twiddle_kernel(float2* __restrict__ tw, int N) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= N) return;
+// Twiddles of the Stockham stages of a length-N FFT, stage-major: the stage with sub-transform
+// length ns (1, 16, 256, ... for the radix-16 stages, then the last stage of radix R = 2, 4 or 8)
+// keeps exp(-2 pi i r m / (ns R)), r = 1 .. R-1, m = 0 .. ns-1, at index r ns + m - 1 (N - 1
+// entries in all).  A butterfly's twiddles for one r are then consecutive across the lanes of a
+// wavefront.  The angle is evaluated as 2 pi k / N with k = r m N / (ns R).
+__global__ void twiddle_kernel(float2* __restrict__ tw, int N, int log2n) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= N - 1) return;
+    const int x = e + 1;        // r ns + m
+    const int n16 = log2n / 4;  // radix-16 stages
+    const int st = min((31 - __clz(x)) / 4, n16);
+    const int ns = 1 << (4 * st);
+    const int R = (st < n16) ? 16 : (1 << (log2n % 4));
+    const int r = x >> (4 * st), m = x & (ns - 1);
+    const int k = r * m * (N / (ns * R));
     double s, c;
     sincos(-2.0 * kPi * k / N, &s, &c);
-    tw[k] = make_float2((float)c, (float)s);
+    tw[e] = make_float2((float)c, (float)s);
 }
 
 __device__ inline float2 cmul(float2 a, float2 b) {
@@ -1818,7 +1830,9 @@ __device__ inline float2 cmul(float2 a, float2 b) {
 __device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 
-constexpr int kFftThreads = 256;
+// Threads per FFT workgroup: 128 up to N = 2048 (one radix-16 butterfly per thread and stage,
+// and twice the workgroups per CU of 256 threads), 256 above (the butterflies' registers)
+constexpr int fft_threads(int log2n) { return log2n <= 11 ? 128 : 256; }
 
 // In-register DFTs of 2, 4, 8 and 16 points (forward: exp(-2 pi i n k / R); `inv` conjugates),
 // results in natural order.  8 = 2 x 4 and 16 = 4 x 4 Cooley-Tukey with the internal twiddles.
@@ -1892,27 +1906,27 @@ __device__ inline void dft_reg(float2 (&v)[R], bool inv) {
 }
 
 // One radix-R Stockham stage over sub-transforms of length ns: butterfly j reads
-// buf[j + r N/R], applies the twiddles exp(-+2 pi i r (j mod ns) / (ns R)), takes an R-point DFT
-// and writes buf[(j / ns) ns R + j mod ns + r ns].  All inputs are read into registers before
-// a barrier, then written, so one LDS buffer suffices.
-template <int R, int N>
+// buf[j + r N/R], applies the twiddles exp(-+2 pi i r (j mod ns) / (ns R)) (stage-major table of
+// twiddle_kernel), takes an R-point DFT and writes buf[(j / ns) ns R + j mod ns + r ns], with T
+// threads per workgroup.  All inputs are read into registers before a barrier, then written, so
+// one LDS buffer suffices.
+template <int R, int N, int T>
 __device__ void stockham_stage(float2* buf, const float2* __restrict__ tw, int ns, bool inv) {
     constexpr int NBF = N / R;  // butterflies
-    constexpr int NB = (NBF + kFftThreads - 1) / kFftThreads;
+    constexpr int NB = (NBF + T - 1) / T;
     const int tid = threadIdx.x;
-    const int tstep = N / (ns * R);
     float2 v[NB][R];
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-        const int j = tid + b * kFftThreads;
+        const int j = tid + b * T;
         if (j < NBF) {
 #pragma unroll
             for (int r = 0; r < R; ++r) v[b][r] = buf[j + r * NBF];
-            const int kt = (j % ns) * tstep;
-            if (kt != 0) {
+            const int m = j % ns;
+            if (m != 0) {
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
-                    float2 w = tw[r * kt];
+                    float2 w = tw[r * ns + m - 1];
                     if (inv) w.y = -w.y;
                     v[b][r] = cmul(v[b][r], w);
                 }
@@ -1923,7 +1937,7 @@ __device__ void stockham_stage(float2* buf, const float2* __restrict__ tw, int n
     __syncthreads();
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-        const int j = tid + b * kFftThreads;
+        const int j = tid + b * T;
         if (j < NBF) {
             const int base = (j / ns) * ns * R + (j % ns);
 #pragma unroll
@@ -1939,18 +1953,19 @@ __device__ void stockham_stage(float2* buf, const float2* __restrict__ tw, int n
 template <int LOG2N>
 __device__ void fft_lds(float2* buf, const float2* __restrict__ tw, bool inv) {
     constexpr int N = 1 << LOG2N;
+    constexpr int T = fft_threads(LOG2N);
     constexpr int kRest = LOG2N % 4;
     int ns = 1;
     if constexpr (LOG2N >= 4) {
 #pragma unroll
         for (int st = 0; st < LOG2N / 4; ++st) {
-            stockham_stage<16, N>(buf, tw, ns, inv);
+            stockham_stage<16, N, T>(buf, tw, ns, inv);
             ns *= 16;
         }
     }
-    if constexpr (kRest == 1) stockham_stage<2, N>(buf, tw, ns, inv);
-    if constexpr (kRest == 2) stockham_stage<4, N>(buf, tw, ns, inv);
-    if constexpr (kRest == 3) stockham_stage<8, N>(buf, tw, ns, inv);
+    if constexpr (kRest == 1) stockham_stage<2, N, T>(buf, tw, ns, inv);
+    if constexpr (kRest == 2) stockham_stage<4, N, T>(buf, tw, ns, inv);
+    if constexpr (kRest == 3) stockham_stage<8, N, T>(buf, tw, ns, inv);
 }
 
 // Filtering() / filter_proj (python/Reconstruction_pycuda.py:223-251): two detector rows per
@@ -1964,18 +1979,19 @@ __device__ void fft_lds(float2* buf, const float2* __restrict__ tw, bool inv) {
 // `in` and `out` may be the same buffer (in-place filtering): a block reads its two rows before
 // its first barrier and writes them after its last one, and no other block touches them.
 template <int LOG2N>
-__global__ void __launch_bounds__(kFftThreads) fdk_filter_kernel(
+__global__ void __launch_bounds__(fft_threads(LOG2N)) fdk_filter_kernel(
     const float* in, float* out, const double* __restrict__ spec,
     const float2* __restrict__ tw, int n_rows, int nu, int nv, float R, float su, float ou,
     float sv, float ov, float out_scale, int mirror) {
     constexpr int N = 1 << LOG2N;
+    constexpr int T = fft_threads(LOG2N);
     __shared__ float2 buf[N];
     const int rowA = 2 * blockIdx.x;
     const int rowB = rowA + 1;
     const bool hasB = rowB < n_rows;
     const float pa = ((float)(nv / 2.0f) - (float)(rowA % nv)) * sv + ov;
     const float pb = ((float)(nv / 2.0f) - (float)(rowB % nv)) * sv + ov;
-    for (int n = threadIdx.x; n < N; n += kFftThreads) {
+    for (int n = threadIdx.x; n < N; n += T) {
         float2 val = make_float2(0.0f, 0.0f);
         if (n < nu) {
             const float kk = ((float)n - (float)nu / 2.0f) * su + ou;
@@ -1990,13 +2006,13 @@ __global__ void __launch_bounds__(kFftThreads) fdk_filter_kernel(
     }
     __syncthreads();
     fft_lds<LOG2N>(buf, tw, false);
-    for (int n = threadIdx.x; n < N; n += kFftThreads) {
+    for (int n = threadIdx.x; n < N; n += T) {
         const float h = (float)spec[n];
         buf[n] = make_float2(buf[n].x * h, buf[n].y * h);
     }
     __syncthreads();
     fft_lds<LOG2N>(buf, tw, true);
-    for (int n = threadIdx.x; n < nu; n += kFftThreads) {
+    for (int n = threadIdx.x; n < nu; n += T) {
         const float2 val = buf[n];
         const int j = mirror ? nu - 1 - n : n;
         out[(size_t)rowA * nu + j] = val.x * out_scale;
@@ -2009,7 +2025,7 @@ hipError_t launch_filter(int l2, dim3 grid, hipStream_t st, const float* in, flo
                          const double* spec, const float2* tw, int n_rows, int nu, int nv, float R,
                          float su, float ou, float sv, float ov, float scale, int mirror) {
     if (l2 == L) {
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(fdk_filter_kernel<L>), grid, dim3(kFftThreads), 0, st,
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(fdk_filter_kernel<L>), grid, dim3(fft_threads(L)), 0, st,
                            in, out, spec, tw, n_rows, nu, nv, R, su, ou, sv, ov, scale, mirror);
         return hipGetLastError();
     }
@@ -2189,7 +2205,7 @@ int ctr_fdk_filter(const ctr_geometry* g, const double* spectrum, const float* p
     if (workspace_bytes < ctr_fdk_filter_workspace_bytes(g)) return CTR_ERR_WORKSPACE;
     hipStream_t s = (hipStream_t)stream;
     float2* tw = (float2*)workspace;
-    hipLaunchKernelGGL(twiddle_kernel, dim3((N + 255) / 256), dim3(256), 0, s, tw, N);
+    hipLaunchKernelGGL(twiddle_kernel, dim3((N + 255) / 256), dim3(256), 0, s, tw, N, l2);
     if ((st = status_from_hip(hipGetLastError()))) return st;
     const int n_rows = n_views * g->nv;
     const double R = g->sad;
@@ -2235,7 +2251,7 @@ int ctr_pixel_filter(const ctr_geometry* g, const double* spectrum, const float*
     if (workspace_bytes < ctr_fdk_filter_workspace_bytes(g)) return CTR_ERR_WORKSPACE;
     hipStream_t s = (hipStream_t)stream;
     float2* tw = (float2*)workspace;
-    hipLaunchKernelGGL(twiddle_kernel, dim3((N + 255) / 256), dim3(256), 0, s, tw, N);
+    hipLaunchKernelGGL(twiddle_kernel, dim3((N + 255) / 256), dim3(256), 0, s, tw, N, l2);
     if ((st = status_from_hip(hipGetLastError()))) return st;
     const int n_rows = n_views * g->nv;
     const double mag = g->sad / g->sdd;

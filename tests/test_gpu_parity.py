@@ -163,6 +163,20 @@ def test_fdk_filter_matches_oracle(dev, ftype, nu):
     assert err <= TOL
 
 
+@pytest.mark.parametrize("nu", [5, 9, 17, 33, 2049, 4097])
+def test_fdk_filter_every_fft_shape(dev, nu):
+    """FFT lengths 8 (one radix-8 stage), 16 (one radix-16 stage), 32 and 64 (radix 16 then 2
+    or 4), 4096 and 8192 (three radix-16 stages, then none or radix 2, on 256-thread groups)."""
+    p = make_params(NumberOfDetectorPixels=[nu, 3], NumberOfViews=2, FilterType="shepp-logan",
+                    DetectorOffset=[0.25, -0.5])
+    g = O.Geometry(p)
+    proj = np.random.default_rng(16).random((2, 3, nu)).astype(np.float32)
+    out = ctr.fdk_filter(torch.from_numpy(proj).to(dev), ctr.ScanGeometry(p)).cpu().numpy()
+    err = _rel(out, O.filter_projections(g, proj))
+    print(f"filter[nu={nu}] max rel err {err:.2e}")
+    assert err <= TOL
+
+
 def test_fdk_filter_in_place_and_odd_row_count(dev):
     p = make_params(NumberOfDetectorPixels=[64, 5], NumberOfViews=3)
     g = O.Geometry(p)
