@@ -1910,6 +1910,9 @@ __device__ inline void dft_reg(float2 (&v)[R], bool inv) {
 // twiddle_kernel), takes an R-point DFT and writes buf[(j / ns) ns R + j mod ns + r ns], with T
 // threads per workgroup.  All inputs are read into registers before a barrier, then written, so
 // one LDS buffer suffices.
+// largest power of two not above r (r >= 1)
+constexpr int top_bit(int r) { return (r & (r - 1)) == 0 ? r : top_bit(r & (r - 1)); }
+
 template <int R, int N, int T>
 __device__ void stockham_stage(float2* buf, const float2* __restrict__ tw, int ns, bool inv) {
     constexpr int NBF = N / R;  // butterflies
@@ -1924,12 +1927,22 @@ __device__ void stockham_stage(float2* buf, const float2* __restrict__ tw, int n
             for (int r = 0; r < R; ++r) v[b][r] = buf[j + r * NBF];
             const int m = j % ns;
             if (m != 0) {
+                // twiddles W^r for r a power of two from the table, the others as products of
+                // two earlier ones (at most three roundings deep): fewer bytes through the
+                // texture path, which is busy nearly all the time in this kernel
+                float2 w[R];
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
-                    float2 w = tw[r * ns + m - 1];
-                    if (inv) w.y = -w.y;
-                    v[b][r] = cmul(v[b][r], w);
+                    if ((r & (r - 1)) == 0) {
+                        w[r] = tw[r * ns + m - 1];
+                        if (inv) w[r].y = -w[r].y;
+                    } else {
+                        const int hb = top_bit(r);
+                        w[r] = cmul(w[hb], w[r - hb]);
+                    }
                 }
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], w[r]);
             }
             dft_reg<R>(v[b], inv);
         }
