@@ -297,13 +297,6 @@ constexpr int kFpMaxQuads = 64 * kFpMaxLoads;
 #define CTR_FP_VIEW_BLOCK 32
 #endif
 constexpr int kViewBlock = CTR_FP_VIEW_BLOCK;
-#ifndef CTR_FP_XCD_RUN
-#define CTR_FP_XCD_RUN 1
-#endif
-constexpr int kXcdRun = CTR_FP_XCD_RUN;  // adjacent column groups per XCD turn
-#ifndef CTR_FP_ORDER
-#define CTR_FP_ORDER 0
-#endif
 
 // two consecutive breakpoints of a z-prefix line
 struct PTaps {
@@ -336,33 +329,19 @@ __global__ void __launch_bounds__(64 * kFpWaves, CTR_FP_MIN_WAVES) dd_forward_ke
     const float* __restrict__ tabs, const float* __restrict__ rows,
     const int32_t* __restrict__ branch, int nx, int ny, int nz, int nzp, int nu, int nv, int stride,
     float zero_tol, int n_views, int col_groups, int n_vblocks) {
-    // workgroup b runs on XCD b % 8, which takes the runs of kXcdRun column groups
-    // cg = (8 c + xcd) kXcdRun + m: every XCD gets an equal share of every band of detector
-    // rows (the rows outside the volume's shadow are cheap).  On one XCD the local index runs
-    // over (row block, view-pair block, c, pair in block, m), fastest last.
+    // workgroup b runs on XCD b % 8, which takes the column groups cg = 8 c + xcd: every XCD
+    // gets an equal share of every band of detector rows (the rows outside the volume's shadow
+    // are cheap).  On one XCD the local index runs over (row block, view-pair block, c, pair in
+    // block), fastest last.
     const int b = blockIdx.x;
     const int xcd = b & 7, l = b >> 3;
-    const int runs = (col_groups + 8 * kXcdRun - 1) / (8 * kXcdRun);
-    // the kXcdRun column groups of a run are consecutive workgroups: they write adjacent parts
-    // of the same detector rows at the same time
-    const int m = l % kXcdRun;
-    const int l2 = l / kXcdRun;
-#if CTR_FP_ORDER == 1
-    const int row_blocks = (nv + kColRows - 1) / kColRows;
-    const int rb = l2 % row_blocks;
-    const int r1 = l2 / row_blocks;
-    const int vi = r1 % kViewBlock;
-    const int r3 = r1 / kViewBlock;
-    const int cg = ((r3 % runs) * 8 + xcd) * kXcdRun + m;
-    const int vp = (r3 / runs) * kViewBlock + vi;  // view pair
-#else
-    const int vi = l2 % kViewBlock;
-    const int r1 = l2 / kViewBlock;
-    const int cg = ((r1 % runs) * 8 + xcd) * kXcdRun + m;
+    const int runs = (col_groups + 7) / 8;
+    const int vi = l % kViewBlock;
+    const int r1 = l / kViewBlock;
+    const int cg = (r1 % runs) * 8 + xcd;
     const int r2 = r1 / runs;
     const int vp = (r2 % n_vblocks) * kViewBlock + vi;  // view pair
     const int rb = r2 / n_vblocks;
-#endif
     const int view0 = kFpViews * vp;
     if (view0 >= n_views || cg >= col_groups) return;
     const int lane = threadIdx.x & 63;
@@ -2167,7 +2146,7 @@ int ctr_dd_forward(const ctr_geometry* g, const double* angles, const int32_t* b
     const int n_pairs = (g->n_views + kFpViews - 1) / kFpViews;
     const int n_vblocks = (n_pairs + kViewBlock - 1) / kViewBlock;
     const long long n_blocks =
-        8LL * kXcdRun * ((col_groups + 8 * kXcdRun - 1) / (8 * kXcdRun)) * row_blocks * n_vblocks * kViewBlock;
+        8LL * ((col_groups + 7) / 8) * row_blocks * n_vblocks * kViewBlock;
     if (n_blocks >= 2147483647LL) return CTR_ERR_INVALID;
     hipLaunchKernelGGL(dd_forward_kernel, dim3((unsigned)n_blocks), dim3(64 * kFpWaves), 0, s, t_ys, t_xs,
                        proj, tabs, rows, branch, g->nx, g->ny, g->nz, nzp, g->nu, g->nv, stride, 1e-6f,
