@@ -153,9 +153,10 @@ def slices_to_input(vol: torch.Tensor, z: torch.Tensor | None = None, in_slices:
 
 
 @torch.no_grad()
-def apply_to_volume(net: ResidualPrior, vol: torch.Tensor, scale: float = 1.0, batch: int = 128) -> torch.Tensor:
+def apply_to_volume(net: ResidualPrior, vol: torch.Tensor, scale: float = 1.0, batch: int = 256) -> torch.Tensor:
     """Run the prior slice by slice over a volume [nz, ny, nx]; inputs are multiplied by ``scale``
-    (network units) and the output divided by it."""
+    (network units) and the output divided by it.  ``batch`` slices go through each layer call
+    (256 slices of 256^2 hold ~2 GB per 64-channel activation)."""
     out = torch.empty_like(vol)
     nz = vol.shape[0]
     for z0 in range(0, nz, batch):
