@@ -7,9 +7,11 @@
 //                         residual, stores bfloat16.  The input gradient of a convolution is
 //                         the same kernel applied to the output gradient with the weights
 //                         flipped in space and transposed in channels.
-//   conv3x3_fwd_persistent_kernel  the same for 64 -> 64 channels: one workgroup per CU with
-//                         all weights in LDS, walking pixel tiles with the next tile's
-//                         input halo loaded while the current one is computed.
+//   conv3x3_fwd_small_cin_kernel  the same for at most 4 input channels, input halo in LDS
+//   conv3x3_fwd_persistent_kernel  the same for 64 input channels and 64 or at most 16 output
+//                         channels: one workgroup per CU with all weights in LDS, walking pixel
+//                         tiles with the next tile's input halo loaded while the current one is
+//                         computed.
 //   conv3x3_wgrad_kernel  weight and bias gradients on MFMA: per (pixel chunk, tap), 64-pixel
 //                         tiles of the output gradient and of the shifted input staged in LDS
 //                         transposed to [channel][pixel]; grad_w = gout^T x_shift with K =
@@ -48,7 +50,7 @@ template <bool FAST>
 __global__ void __launch_bounds__(256) conv3x3_fwd_kernel(
     const bf16_t* __restrict__ in, const bf16_t* __restrict__ w, const float* __restrict__ bias,
     const bf16_t* __restrict__ resid, bf16_t* __restrict__ out, int N, int H, int W, int Cin,
-    int Cout, int relu) {
+    int Cout, int relu, int vec) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int P = N * H * W;
     const int pbase = (blockIdx.x * kConvWaves + wave) * kPxPerWave;
@@ -118,36 +120,167 @@ __global__ void __launch_bounds__(256) conv3x3_fwd_kernel(
                 }
             }
         }
+        // the weights are the row operand: lane (m, g) receives output channels
+        // 16 t + 4 g .. +3 of pixel m, four adjacent values of one NHWC row
 #pragma unroll
         for (int h = 0; h < 2; ++h)
 #pragma unroll
             for (int t = 0; t < 4; ++t)
                 if (t < n_tiles)  // uniform: column tiles past Cout are all zero
-                    acc[h][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[h], b[t], acc[h][t], 0, 0, 0);
+                    acc[h][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[t], a[h], acc[h][t], 0, 0, 0);
     }
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-        const int co = t * 16 + m;
-        if (co >= Cout) continue;
-        const float bc = bias ? bias[co] : 0.0f;
+        if (t >= n_tiles) continue;
+        const int co0 = t * 16 + 4 * g;
+        if (co0 >= Cout) continue;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
+            const int p = pbase + h * 16 + m;
+            if (p >= P) continue;
+            const size_t o = (size_t)p * Cout + co0;
+            float v[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int p = pbase + h * 16 + 4 * g + r;
-                if (p >= P) continue;
-                float v = acc[h][t][r] + bc;
-                if (relu) v = fmaxf(v, 0.0f);
-                const size_t o = (size_t)p * Cout + co;
-                if (resid) v += (float)resid[o];
-                out[o] = (bf16_t)v;
+                v[r] = acc[h][t][r] + ((bias && co0 + r < Cout) ? bias[co0 + r] : 0.0f);
+                if (relu) v[r] = fmaxf(v[r], 0.0f);
+            }
+            if (vec) {  // Cout % 4 == 0 and 8-byte aligned rows: four channels per access
+                if (resid) {
+                    const bf16x4 rv = *reinterpret_cast<const bf16x4*>(resid + o);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) v[r] += (float)rv[r];
+                }
+                bf16x4 q;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) q[r] = (bf16_t)v[r];
+                *reinterpret_cast<bf16x4*>(out + o) = q;
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    if (co0 + r >= Cout) break;
+                    float x = v[r];
+                    if (resid) x += (float)resid[o + r];
+                    out[o + r] = (bf16_t)x;
+                }
             }
         }
     }
 }
 
-// Persistent LDS variant for the 64 -> 64 channel body of the network.  One 512-thread
-// workgroup per CU keeps all nine taps of the weights in LDS (72 KB) and walks 16 x 32 pixel
+// Convolutions with at most 4 input channels (the network's first layer, 3 -> 64, and the input
+// gradient of its last, 1 -> 64).  K = 9 taps x 4 channels (zero past Cin) in two MFMA K steps:
+// taps 0-7, then tap 8.  A workgroup computes an 8 x 32 pixel tile from its (8+2) x (32+2) input
+// halo in LDS (one 8-byte entry of 4 channels per pixel); lane (m, g) of wave w takes pixel m of
+// the four 16-pixel row tiles of rows 2w, 2w+1 with K elements 8g .. 8g+7 (taps 2g and 2g+1 in
+// the first step), and the zero-padded weights stay in registers for the whole tile.
+constexpr int kSTileH = 8, kSTileW = 32;
+constexpr int kSHaloW = kSTileW + 2;
+constexpr int kSHaloPix = (kSTileH + 2) * kSHaloW;
+
+__global__ void __launch_bounds__(256) conv3x3_fwd_small_cin_kernel(
+    const bf16_t* __restrict__ in, const bf16_t* __restrict__ w, const float* __restrict__ bias,
+    const bf16_t* __restrict__ resid, bf16_t* __restrict__ out, int H, int W, int Cin, int Cout,
+    int relu, int vec, int tiles_x, int tiles_y) {
+    __shared__ __attribute__((aligned(16))) bf16x4 halo[kSHaloPix];
+    const int tid = threadIdx.x;
+    const int tile = blockIdx.x;
+    const int tx = tile % tiles_x, ty = (tile / tiles_x) % tiles_y, n = tile / (tiles_x * tiles_y);
+    const int ty0 = ty * kSTileH, tx0 = tx * kSTileW;
+    for (int e = tid; e < kSHaloPix; e += 256) {
+        const int yy = ty0 + e / kSHaloW - 1, xx = tx0 + e % kSHaloW - 1;
+        bf16x4 v;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) v[c] = (bf16_t)0.0f;
+        if ((unsigned)yy < (unsigned)H && (unsigned)xx < (unsigned)W) {
+            const bf16_t* src = in + ((size_t)(n * H + yy) * W + xx) * Cin;
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                if (c < Cin) v[c] = src[c];
+        }
+        halo[e] = v;
+    }
+    const int lane = tid & 63, wave = tid >> 6;
+    const int m = lane & 15, g = lane >> 4;
+    const int n_ct = (Cout + 15) / 16;  // column tiles (uniform)
+    // weights as the row operand: lane (m, g) of column tile t holds output channel 16 t + m,
+    // K elements k = 32 s + 8 g + e of step s (tap k / 4, channel k % 4)
+    bf16x8 b[2][4];
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int k = 32 * s + 8 * g + e, tap = k >> 2, ci = k & 3, co = 16 * t + m;
+                b[s][t][e] = (tap < 9 && ci < Cin && co < Cout) ? w[((size_t)co * 9 + tap) * Cin + ci] : (bf16_t)0.0f;
+            }
+    // lane (m, g) receives channels 16 t + 4 g .. +3 of its pixel: the bias starts them
+    float bias_t[4][4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int co = 16 * t + 4 * g + r;
+            bias_t[t][r] = (bias && co < Cout) ? bias[co] : 0.0f;
+        }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int row = 2 * wave + (i >> 1), col = 16 * (i & 1) + m;
+        // halo entry of tap (dy, dx) for this pixel: (row + dy) * kSHaloW + col + dx
+        const int t0 = 2 * g, t1 = 2 * g + 1;
+        const bf16x4 p0 = halo[(row + t0 / 3) * kSHaloW + col + t0 % 3];
+        const bf16x4 p1 = halo[(row + t1 / 3) * kSHaloW + col + t1 % 3];
+        const bf16x4 p8 = halo[(row + 2) * kSHaloW + col + 2];
+        bf16x8 a0, a1;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            a0[e] = p0[e];
+            a0[4 + e] = p1[e];
+            a1[e] = g == 0 ? p8[e] : (bf16_t)0.0f;
+            a1[4 + e] = (bf16_t)0.0f;
+        }
+        const int yy = ty0 + row, xx = tx0 + col;
+        const bool ok = yy < H && xx < W;
+        const size_t o = ok ? ((size_t)(n * H + yy) * W + xx) * Cout : 0;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            if (t >= n_ct) break;
+            f32x4 acc = f32x4{bias_t[t][0], bias_t[t][1], bias_t[t][2], bias_t[t][3]};
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[0][t], a0, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[1][t], a1, acc, 0, 0, 0);
+            const int co0 = 16 * t + 4 * g;
+            if (!ok || co0 >= Cout) continue;
+            float v[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) v[r] = relu ? fmaxf(acc[r], 0.0f) : acc[r];
+            if (vec) {  // Cout % 4 == 0 and 8-byte aligned rows
+                if (resid) {
+                    const bf16x4 rv = *reinterpret_cast<const bf16x4*>(resid + o + co0);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) v[r] += (float)rv[r];
+                }
+                bf16x4 q;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) q[r] = (bf16_t)v[r];
+                *reinterpret_cast<bf16x4*>(out + o + co0) = q;
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    if (co0 + r >= Cout) break;
+                    float x = v[r];
+                    if (resid) x += (float)resid[o + co0 + r];
+                    out[o + co0 + r] = (bf16_t)x;
+                }
+            }
+        }
+    }
+}
+
+// Persistent LDS variant for 64 input channels: the 64 -> 64 body of the network and its last
+// layer (64 -> 1).  One 512-thread workgroup per CU keeps all nine taps of the weights in LDS
+// (72 KB for 64 output channels) and walks 16 x 32 pixel
 // tiles: the (16+2) x (32+2) input halo of the current tile sits in LDS (76.5 KB) while the
 // next tile's halo is already loaded into registers.  Wave w computes rows 2w, 2w+1 of the
 // tile: 4 MFMA row tiles (16 pixels) x 4 column tiles (16 channels), 16 MFMAs per 8 LDS
@@ -163,21 +296,26 @@ constexpr int kPPer = (kPHaloChunks + kPThreads - 1) / kPThreads;
 
 __device__ inline int swz(int row, int chunk) { return chunk ^ ((row >> 1) & 7); }
 
-template <bool RELU, bool RESID>
+// NT = 4 column tiles: 64 output channels, staged epilogue with 16-byte stores (and optional
+// residual); NT = 1: up to 16 output channels (weights past Cout are zero), each lane stores its
+// own channels (the network's last layer, 64 -> 1).
+template <bool RELU, bool RESID, int NT>
 __global__ void __launch_bounds__(kPThreads) conv3x3_fwd_persistent_kernel(
     const bf16_t* __restrict__ in, const bf16_t* __restrict__ w, const float* __restrict__ bias,
-    const bf16_t* __restrict__ resid, bf16_t* __restrict__ out, int H, int W, int tiles_x,
+    const bf16_t* __restrict__ resid, bf16_t* __restrict__ out, int H, int W, int Cout, int tiles_x,
     int tiles_y, int n_tiles) {
+    static_assert(NT == 4 || (NT == 1 && !RESID), "64 channels, or at most 16 without residual");
     constexpr int K = 9 * 64;
-    __shared__ __attribute__((aligned(16))) bf16_t wl[64 * 9 * 64];
+    constexpr int kCo = 16 * NT;  // output channels held in LDS (zero past Cout)
+    __shared__ __attribute__((aligned(16))) bf16_t wl[kCo * 9 * 64];
     __shared__ __attribute__((aligned(16))) bf16_t halo[kPHaloPix * 64];
     const int tid = threadIdx.x;
     // all weights: row (co, tap) of 8 chunks
-    for (int e = tid; e < 64 * 9 * 8; e += kPThreads) {
+    for (int e = tid; e < kCo * 9 * 8; e += kPThreads) {
         const int c = e & 7, row = e >> 3;  // row = co * 9 + tap
         const int co = row / 9;
         *reinterpret_cast<bf16x8*>(&wl[(row * 8 + swz(co, c)) * 8]) =
-            *reinterpret_cast<const bf16x8*>(w + (size_t)co * K + (row - co * 9) * 64 + c * 8);
+            co < Cout ? *reinterpret_cast<const bf16x8*>(w + (size_t)co * K + (row - co * 9) * 64 + c * 8) : zero8();
     }
     // the halo chunks of this thread, the same for every tile: position in the halo relative to
     // the tile origin (hy, hx >= -1), element offset in the input, element offset in LDS
@@ -219,11 +357,14 @@ __global__ void __launch_bounds__(kPThreads) conv3x3_fwd_persistent_kernel(
     const int m = lane & 15, g = lane >> 4;
     // the MFMAs take the weights as the row operand, so lane (m, g) holds output channels
     // 16 t + 4 g + r of pixel m: the bias of those channels starts every accumulator
-    float bias_t[4][4];
+    float bias_t[NT][4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
+    for (int t = 0; t < NT; ++t)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) bias_t[t][r] = bias ? bias[t * 16 + 4 * g + r] : 0.0f;
+        for (int r = 0; r < 4; ++r) {
+            const int co = t * 16 + 4 * g + r;
+            bias_t[t][r] = (bias && co < Cout) ? bias[co] : 0.0f;
+        }
     int tile = blockIdx.x;
     if (tile < n_tiles) {
         fetch(tile);
@@ -232,11 +373,11 @@ __global__ void __launch_bounds__(kPThreads) conv3x3_fwd_persistent_kernel(
     __syncthreads();
     while (tile < n_tiles) {
         const int next = tile + (int)gridDim.x;
-        f32x4 acc[4][4];
+        f32x4 acc[4][NT];
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
-            for (int t = 0; t < 4; ++t) acc[i][t] = f32x4{bias_t[t][0], bias_t[t][1], bias_t[t][2], bias_t[t][3]};
+            for (int t = 0; t < NT; ++t) acc[i][t] = f32x4{bias_t[t][0], bias_t[t][1], bias_t[t][2], bias_t[t][3]};
         // the previous tile's stores have completed (their data registers are reused by the
         // accumulators); the next tile's halo is then in flight while this tile is computed
         __builtin_amdgcn_s_waitcnt(0);
@@ -247,9 +388,9 @@ __global__ void __launch_bounds__(kPThreads) conv3x3_fwd_persistent_kernel(
 #pragma unroll
             for (int c0 = 0; c0 < 8; c0 += 4) {  // chunk offset of the K step (32 channels)
                 const int ch = c0 + g;
-                bf16x8 b[4], a[4];
+                bf16x8 b[NT], a[4];
 #pragma unroll
-                for (int t = 0; t < 4; ++t) {
+                for (int t = 0; t < NT; ++t) {
                     const int co = t * 16 + m;
                     b[t] = *reinterpret_cast<const bf16x8*>(&wl[((co * 9 + tap) * 8 + swz(co, ch)) * 8]);
                 }
@@ -261,28 +402,45 @@ __global__ void __launch_bounds__(kPThreads) conv3x3_fwd_persistent_kernel(
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
 #pragma unroll
-                    for (int t = 0; t < 4; ++t)
+                    for (int t = 0; t < NT; ++t)
                         acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[t], a[i], acc[i][t], 0, 0, 0);
             }
         }
-        __syncthreads();  // every wave is done with this halo: reuse it to stage the results
-        // (ReLU), as bfloat16, into stage[pixel][channel] (pixel = row * 32 + column, rows of 8
-        // 16-byte chunks swizzled like the halo): four channels per 8-byte store
-        bf16_t* stage = halo;
+        if constexpr (NT == 1) {
+            // lane (m, g) holds channels 4 g .. 4 g + 3 of pixel m of each row tile: stored
+            // straight from the accumulators (at most 16 channels per pixel)
+            const int tx = tile % tiles_x;
+            const int ty = (tile / tiles_x) % tiles_y;
+            const int n = tile / (tiles_x * tiles_y);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int p = (2 * wave + (i >> 1)) * kPTileW + 16 * (i & 1) + m;
+            for (int i = 0; i < 4; ++i) {
+                const int yy = ty * kPTileH + 2 * wave + (i >> 1), xx = tx * kPTileW + 16 * (i & 1) + m;
+                if (yy >= H || xx >= W) continue;
+                const size_t o = ((size_t)(n * H + yy) * W + xx) * Cout;
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int co = t * 16 + 4 * g;
-                bf16x4 q;
-#pragma unroll
-                for (int r = 0; r < 4; ++r) q[r] = (bf16_t)(RELU ? fmaxf(acc[i][t][r], 0.0f) : acc[i][t][r]);
-                *reinterpret_cast<bf16x4*>(&stage[(p * 8 + swz(p, co >> 3)) * 8 + (co & 7)]) = q;
+                for (int r = 0; r < 4; ++r) {
+                    const int co = 4 * g + r;
+                    if (co < Cout) out[o + co] = (bf16_t)(RELU ? fmaxf(acc[i][0][r], 0.0f) : acc[i][0][r]);
+                }
             }
-        }
-        __syncthreads();
-        {
+        } else {
+            __syncthreads();  // every wave is done with this halo: reuse it to stage the results
+            // (ReLU), as bfloat16, into stage[pixel][channel] (pixel = row * 32 + column, rows of 8
+            // 16-byte chunks swizzled like the halo): four channels per 8-byte store
+            bf16_t* stage = halo;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int p = (2 * wave + (i >> 1)) * kPTileW + 16 * (i & 1) + m;
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int co = t * 16 + 4 * g;
+                    bf16x4 q;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) q[r] = (bf16_t)(RELU ? fmaxf(acc[i][t][r], 0.0f) : acc[i][t][r]);
+                    *reinterpret_cast<bf16x4*>(&stage[(p * 8 + swz(p, co >> 3)) * 8 + (co & 7)]) = q;
+                }
+            }
+            __syncthreads();
             const int tx = tile % tiles_x;
             const int ty = (tile / tiles_x) % tiles_y;
             const int n = tile / (tiles_x * tiles_y);
@@ -463,7 +621,9 @@ int ctr_conv3x3_bf16_forward(const void* in, const void* weight, const float* bi
     const bf16_t* wt = (const bf16_t*)weight;
     const bf16_t* r = (const bf16_t*)residual;
     bf16_t* y = (bf16_t*)out;
-    if (cout == 64 && cin == 64) {
+    // the generic kernel stores four channels at a time when rows allow it
+    const int vec = (cout % 4 == 0) && ((reinterpret_cast<uintptr_t>(out) | reinterpret_cast<uintptr_t>(residual)) & 7) == 0;
+    if (cin == 64 && (cout == 64 || (cout <= 16 && !r))) {
         const int tiles_x = (w + kPTileW - 1) / kPTileW, tiles_y = (h + kPTileH - 1) / kPTileH;
         const long long n_tiles = (long long)n * tiles_x * tiles_y;
         if (n_tiles >= 2147483647LL) return CTR_ERR_INVALID;
@@ -473,19 +633,27 @@ int ctr_conv3x3_bf16_forward(const void* in, const void* weight, const float* bi
             return CTR_ERR_HIP;
         const int grid = (int)std::min<long long>(n_tiles, std::max(n_cu, 1));
         auto launch = [&](auto kern) {
-            hipLaunchKernelGGL(kern, dim3(grid), dim3(kPThreads), 0, s, x, wt, bias, r, y, h, w, tiles_x,
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(kPThreads), 0, s, x, wt, bias, r, y, h, w, cout, tiles_x,
                                tiles_y, (int)n_tiles);
         };
-        if (relu)
-            r ? launch(conv3x3_fwd_persistent_kernel<true, true>) : launch(conv3x3_fwd_persistent_kernel<true, false>);
+        if (cout <= 16)
+            relu ? launch(conv3x3_fwd_persistent_kernel<true, false, 1>) : launch(conv3x3_fwd_persistent_kernel<false, false, 1>);
+        else if (relu)
+            r ? launch(conv3x3_fwd_persistent_kernel<true, true, 4>) : launch(conv3x3_fwd_persistent_kernel<true, false, 4>);
         else
-            r ? launch(conv3x3_fwd_persistent_kernel<false, true>) : launch(conv3x3_fwd_persistent_kernel<false, false>);
+            r ? launch(conv3x3_fwd_persistent_kernel<false, true, 4>) : launch(conv3x3_fwd_persistent_kernel<false, false, 4>);
+    } else if (cin <= 4) {
+        const int tiles_x = (w + kSTileW - 1) / kSTileW, tiles_y = (h + kSTileH - 1) / kSTileH;
+        const long long n_tiles = (long long)n * tiles_x * tiles_y;
+        if (n_tiles >= 2147483647LL) return CTR_ERR_INVALID;
+        hipLaunchKernelGGL(conv3x3_fwd_small_cin_kernel, dim3((unsigned)n_tiles), dim3(256), 0, s, x, wt, bias, r, y,
+                           h, w, cin, cout, relu, vec, tiles_x, tiles_y);
     } else if (cin % 32 == 0) {
         hipLaunchKernelGGL(HIP_KERNEL_NAME(conv3x3_fwd_kernel<true>), dim3((unsigned)blocks), dim3(256),
-                           0, s, x, wt, bias, r, y, n, h, w, cin, cout, relu);
+                           0, s, x, wt, bias, r, y, n, h, w, cin, cout, relu, vec);
     } else {
         hipLaunchKernelGGL(HIP_KERNEL_NAME(conv3x3_fwd_kernel<false>), dim3((unsigned)blocks), dim3(256),
-                           0, s, x, wt, bias, r, y, n, h, w, cin, cout, relu);
+                           0, s, x, wt, bias, r, y, n, h, w, cin, cout, relu, vec);
     }
     return hipGetLastError() == hipSuccess ? CTR_OK : CTR_ERR_HIP;
 }

@@ -44,14 +44,24 @@ def test_conv_forward_matches_cpu(dev, n, h, w, cin, cout):
     assert err <= 2 ** -7  # bfloat16 output rounding
 
 
-def test_conv_forward_fused_epilogue(dev):
-    g = torch.Generator().manual_seed(7)
-    x = torch.randn((2, 16, 16, 64), generator=g).to(torch.bfloat16)
-    wt = (torch.randn((64, 3, 3, 64), generator=g) * 0.05).to(torch.bfloat16)
-    b = torch.randn(64, generator=g)
-    r = torch.randn((2, 16, 16, 64), generator=g).to(torch.bfloat16)
-    out = PR.conv3x3_forward(x.to(dev), wt.to(dev), b.to(dev), relu=True, residual=r.to(dev)).float().cpu()
-    ref = torch.relu(ref_conv(x, wt, b)) + r.float()
+@pytest.mark.parametrize("cin,cout,relu,resid", [(64, 64, True, True), (3, 64, True, False),
+                                                 (1, 64, False, True), (64, 16, True, False),
+                                                 (4, 8, True, True), (64, 1, False, False)])
+def test_conv_forward_fused_epilogue(dev, cin, cout, relu, resid):
+    """Bias, ReLU and residual in each kernel's epilogue: the persistent kernel (64 -> 64 and
+    64 -> at most 16), the small-input kernel (at most 4 channels in) and the generic one."""
+    g = torch.Generator().manual_seed(7 + cin + cout)
+    x = torch.randn((2, 19, 37, cin), generator=g).to(torch.bfloat16)
+    wt = (torch.randn((cout, 3, 3, cin), generator=g) * (0.3 / cin ** 0.5)).to(torch.bfloat16)
+    b = torch.randn(cout, generator=g)
+    r = torch.randn((2, 19, 37, cout), generator=g).to(torch.bfloat16) if resid else None
+    out = PR.conv3x3_forward(x.to(dev), wt.to(dev), b.to(dev), relu=relu,
+                             residual=r.to(dev) if resid else None).float().cpu()
+    ref = ref_conv(x, wt, b)
+    if relu:
+        ref = torch.relu(ref)
+    if resid:
+        ref = ref + r.float()
     assert (out - ref).abs().max().item() <= 2 ** -7 * ref.abs().max().item()
 
 
