@@ -424,6 +424,24 @@ __global__ void __launch_bounds__(kPThreads) conv3x3_fwd_persistent_kernel(
                 }
             }
         } else {
+            const int tx = tile % tiles_x;
+            const int ty = (tile / tiles_x) % tiles_y;
+            const int n = tile / (tiles_x * tiles_y);
+            // this thread's output chunks (pixel, 8 channels): their residual is loaded now, so
+            // that it arrives while the results are staged
+            constexpr int kChunks = kPTileH * kPTileW * 8 / kPThreads;
+            bf16x8 rv[RESID ? kChunks : 1];
+            size_t o[kChunks];
+            bool ok[kChunks];
+#pragma unroll
+            for (int k = 0; k < kChunks; ++k) {
+                const int e = tid + k * kPThreads;
+                const int c = e & 7, p = e >> 3;
+                const int yy = ty * kPTileH + p / kPTileW, xx = tx * kPTileW + p % kPTileW;
+                ok[k] = yy < H && xx < W;
+                o[k] = ((size_t)(n * H + min(yy, H - 1)) * W + min(xx, W - 1)) * 64 + c * 8;
+                if constexpr (RESID) rv[k] = *reinterpret_cast<const bf16x8*>(resid + o[k]);
+            }
             __syncthreads();  // every wave is done with this halo: reuse it to stage the results
             // (ReLU), as bfloat16, into stage[pixel][channel] (pixel = row * 32 + column, rows of 8
             // 16-byte chunks swizzled like the halo): four channels per 8-byte store
@@ -441,37 +459,17 @@ __global__ void __launch_bounds__(kPThreads) conv3x3_fwd_persistent_kernel(
                 }
             }
             __syncthreads();
-            const int tx = tile % tiles_x;
-            const int ty = (tile / tiles_x) % tiles_y;
-            const int n = tile / (tiles_x * tiles_y);
-            // 16-byte chunks (pixel, 8 channels), in two halves of four per thread: the residual
-            // chunks of a half are all loaded before any is used; residual added in FP32
-            constexpr int kChunks = kPTileH * kPTileW * 8 / kPThreads;
+            // 16-byte chunks (pixel, 8 channels); residual added in FP32
 #pragma unroll
-            for (int h0 = 0; h0 < kChunks; h0 += 4) {
-                bf16x8 rv[4];
-                size_t o[4];
-                bool ok[4];
+            for (int k = 0; k < kChunks; ++k) {
+                const int e = tid + k * kPThreads;
+                const int c = e & 7, p = e >> 3;
+                bf16x8 v = *reinterpret_cast<const bf16x8*>(&stage[(p * 8 + swz(p, c)) * 8]);
+                if constexpr (RESID) {
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const int e = tid + (h0 + k) * kPThreads;
-                    const int c = e & 7, p = e >> 3;
-                    const int yy = ty * kPTileH + p / kPTileW, xx = tx * kPTileW + p % kPTileW;
-                    ok[k] = yy < H && xx < W;
-                    o[k] = ((size_t)(n * H + min(yy, H - 1)) * W + min(xx, W - 1)) * 64 + c * 8;
-                    if constexpr (RESID) rv[k] = *reinterpret_cast<const bf16x8*>(resid + o[k]);
+                    for (int q = 0; q < 8; ++q) v[q] = (bf16_t)((float)v[q] + (float)rv[k][q]);
                 }
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const int e = tid + (h0 + k) * kPThreads;
-                    const int c = e & 7, p = e >> 3;
-                    bf16x8 v = *reinterpret_cast<const bf16x8*>(&stage[(p * 8 + swz(p, c)) * 8]);
-                    if constexpr (RESID) {
-#pragma unroll
-                        for (int q = 0; q < 8; ++q) v[q] = (bf16_t)((float)v[q] + (float)rv[k][q]);
-                    }
-                    if (ok[k]) *reinterpret_cast<bf16x8*>(out + o[k]) = v;
-                }
+                if (ok[k]) *reinterpret_cast<bf16x8*>(out + o[k]) = v;
             }
         }
         __syncthreads();  // every wave is done with this halo
