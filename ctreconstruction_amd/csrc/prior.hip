@@ -173,7 +173,8 @@ __global__ void __launch_bounds__(256) conv3x3_fwd_kernel(
 // taps 0-7, then tap 8.  A workgroup computes an 8 x 32 pixel tile from its (8+2) x (32+2) input
 // halo in LDS (one 8-byte entry of 4 channels per pixel); lane (m, g) of wave w takes pixel m of
 // the four 16-pixel row tiles of rows 2w, 2w+1 with K elements 8g .. 8g+7 (taps 2g and 2g+1 in
-// the first step), and the zero-padded weights stay in registers for the whole tile.
+// the first step).  The workgroups are persistent: each builds the zero-padded weight operands
+// in registers once and then walks tiles.
 constexpr int kSTileH = 8, kSTileW = 32;
 constexpr int kSHaloW = kSTileW + 2;
 constexpr int kSHaloPix = (kSTileH + 2) * kSHaloW;
@@ -181,25 +182,9 @@ constexpr int kSHaloPix = (kSTileH + 2) * kSHaloW;
 __global__ void __launch_bounds__(256) conv3x3_fwd_small_cin_kernel(
     const bf16_t* __restrict__ in, const bf16_t* __restrict__ w, const float* __restrict__ bias,
     const bf16_t* __restrict__ resid, bf16_t* __restrict__ out, int H, int W, int Cin, int Cout,
-    int relu, int vec, int tiles_x, int tiles_y) {
+    int relu, int vec, int tiles_x, int tiles_y, int n_tiles) {
     __shared__ __attribute__((aligned(16))) bf16x4 halo[kSHaloPix];
     const int tid = threadIdx.x;
-    const int tile = blockIdx.x;
-    const int tx = tile % tiles_x, ty = (tile / tiles_x) % tiles_y, n = tile / (tiles_x * tiles_y);
-    const int ty0 = ty * kSTileH, tx0 = tx * kSTileW;
-    for (int e = tid; e < kSHaloPix; e += 256) {
-        const int yy = ty0 + e / kSHaloW - 1, xx = tx0 + e % kSHaloW - 1;
-        bf16x4 v;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) v[c] = (bf16_t)0.0f;
-        if ((unsigned)yy < (unsigned)H && (unsigned)xx < (unsigned)W) {
-            const bf16_t* src = in + ((size_t)(n * H + yy) * W + xx) * Cin;
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
-                if (c < Cin) v[c] = src[c];
-        }
-        halo[e] = v;
-    }
     const int lane = tid & 63, wave = tid >> 6;
     const int m = lane & 15, g = lane >> 4;
     const int n_ct = (Cout + 15) / 16;  // column tiles (uniform)
@@ -224,54 +209,72 @@ __global__ void __launch_bounds__(256) conv3x3_fwd_small_cin_kernel(
             const int co = 16 * t + 4 * g + r;
             bias_t[t][r] = (bias && co < Cout) ? bias[co] : 0.0f;
         }
-    __syncthreads();
+    for (int tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+        const int tx = tile % tiles_x, ty = (tile / tiles_x) % tiles_y, n = tile / (tiles_x * tiles_y);
+        const int ty0 = ty * kSTileH, tx0 = tx * kSTileW;
+        __syncthreads();  // the previous tile's halo is read
+        for (int e = tid; e < kSHaloPix; e += 256) {
+            const int yy = ty0 + e / kSHaloW - 1, xx = tx0 + e % kSHaloW - 1;
+            bf16x4 v;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int row = 2 * wave + (i >> 1), col = 16 * (i & 1) + m;
-        // halo entry of tap (dy, dx) for this pixel: (row + dy) * kSHaloW + col + dx
-        const int t0 = 2 * g, t1 = 2 * g + 1;
-        const bf16x4 p0 = halo[(row + t0 / 3) * kSHaloW + col + t0 % 3];
-        const bf16x4 p1 = halo[(row + t1 / 3) * kSHaloW + col + t1 % 3];
-        const bf16x4 p8 = halo[(row + 2) * kSHaloW + col + 2];
-        bf16x8 a0, a1;
+            for (int c = 0; c < 4; ++c) v[c] = (bf16_t)0.0f;
+            if ((unsigned)yy < (unsigned)H && (unsigned)xx < (unsigned)W) {
+                const bf16_t* src = in + ((size_t)(n * H + yy) * W + xx) * Cin;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            a0[e] = p0[e];
-            a0[4 + e] = p1[e];
-            a1[e] = g == 0 ? p8[e] : (bf16_t)0.0f;
-            a1[4 + e] = (bf16_t)0.0f;
+                for (int c = 0; c < 4; ++c)
+                    if (c < Cin) v[c] = src[c];
+            }
+            halo[e] = v;
         }
-        const int yy = ty0 + row, xx = tx0 + col;
-        const bool ok = yy < H && xx < W;
-        const size_t o = ok ? ((size_t)(n * H + yy) * W + xx) * Cout : 0;
+        __syncthreads();
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            if (t >= n_ct) break;
-            f32x4 acc = f32x4{bias_t[t][0], bias_t[t][1], bias_t[t][2], bias_t[t][3]};
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[0][t], a0, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[1][t], a1, acc, 0, 0, 0);
-            const int co0 = 16 * t + 4 * g;
-            if (!ok || co0 >= Cout) continue;
-            float v[4];
+        for (int i = 0; i < 4; ++i) {
+            const int row = 2 * wave + (i >> 1), col = 16 * (i & 1) + m;
+            // halo entry of tap (dy, dx) for this pixel: (row + dy) * kSHaloW + col + dx
+            const int t0 = 2 * g, t1 = 2 * g + 1;
+            const bf16x4 p0 = halo[(row + t0 / 3) * kSHaloW + col + t0 % 3];
+            const bf16x4 p1 = halo[(row + t1 / 3) * kSHaloW + col + t1 % 3];
+            const bf16x4 p8 = halo[(row + 2) * kSHaloW + col + 2];
+            bf16x8 a0, a1;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) v[r] = relu ? fmaxf(acc[r], 0.0f) : acc[r];
-            if (vec) {  // Cout % 4 == 0 and 8-byte aligned rows
-                if (resid) {
-                    const bf16x4 rv = *reinterpret_cast<const bf16x4*>(resid + o + co0);
+            for (int e = 0; e < 4; ++e) {
+                a0[e] = p0[e];
+                a0[4 + e] = p1[e];
+                a1[e] = g == 0 ? p8[e] : (bf16_t)0.0f;
+                a1[4 + e] = (bf16_t)0.0f;
+            }
+            const int yy = ty0 + row, xx = tx0 + col;
+            const bool ok = yy < H && xx < W;
+            const size_t o = ok ? ((size_t)(n * H + yy) * W + xx) * Cout : 0;
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) v[r] += (float)rv[r];
-                }
-                bf16x4 q;
+            for (int t = 0; t < 4; ++t) {
+                if (t >= n_ct) break;
+                f32x4 acc = f32x4{bias_t[t][0], bias_t[t][1], bias_t[t][2], bias_t[t][3]};
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[0][t], a0, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[1][t], a1, acc, 0, 0, 0);
+                const int co0 = 16 * t + 4 * g;
+                if (!ok || co0 >= Cout) continue;
+                float v[4];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) q[r] = (bf16_t)v[r];
-                *reinterpret_cast<bf16x4*>(out + o + co0) = q;
-            } else {
+                for (int r = 0; r < 4; ++r) v[r] = relu ? fmaxf(acc[r], 0.0f) : acc[r];
+                if (vec) {  // Cout % 4 == 0 and 8-byte aligned rows
+                    if (resid) {
+                        const bf16x4 rv = *reinterpret_cast<const bf16x4*>(resid + o + co0);
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    if (co0 + r >= Cout) break;
-                    float x = v[r];
-                    if (resid) x += (float)resid[o + co0 + r];
-                    out[o + co0 + r] = (bf16_t)x;
+                        for (int r = 0; r < 4; ++r) v[r] += (float)rv[r];
+                    }
+                    bf16x4 q;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) q[r] = (bf16_t)v[r];
+                    *reinterpret_cast<bf16x4*>(out + o + co0) = q;
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        if (co0 + r >= Cout) break;
+                        float x = v[r];
+                        if (resid) x += (float)resid[o + co0 + r];
+                        out[o + co0 + r] = (bf16_t)x;
+                    }
                 }
             }
         }
@@ -644,8 +647,14 @@ int ctr_conv3x3_bf16_forward(const void* in, const void* weight, const float* bi
         const int tiles_x = (w + kSTileW - 1) / kSTileW, tiles_y = (h + kSTileH - 1) / kSTileH;
         const long long n_tiles = (long long)n * tiles_x * tiles_y;
         if (n_tiles >= 2147483647LL) return CTR_ERR_INVALID;
-        hipLaunchKernelGGL(conv3x3_fwd_small_cin_kernel, dim3((unsigned)n_tiles), dim3(256), 0, s, x, wt, bias, r, y,
-                           h, w, cin, cout, relu, vec, tiles_x, tiles_y);
+        int dev = 0, n_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return CTR_ERR_HIP;
+        // persistent: eight workgroups per CU walk the tiles
+        const int grid = (int)std::min<long long>(n_tiles, 8LL * std::max(n_cu, 1));
+        hipLaunchKernelGGL(conv3x3_fwd_small_cin_kernel, dim3(grid), dim3(256), 0, s, x, wt, bias, r, y,
+                           h, w, cin, cout, relu, vec, tiles_x, tiles_y, (int)n_tiles);
     } else if (cin % 32 == 0) {
         hipLaunchKernelGGL(HIP_KERNEL_NAME(conv3x3_fwd_kernel<true>), dim3((unsigned)blocks), dim3(256),
                            0, s, x, wt, bias, r, y, n, h, w, cin, cout, relu, vec);
