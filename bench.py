@@ -353,7 +353,7 @@ def main(argv=None):
                     help="also run the instruction-issue microbenchmarks (VALU ceiling calibration)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; the view-sharded multi-GPU layout is "
+        print(json.dumps({"error": "bench.py runs on one GPU; the view-sharded multi-GPU layout is "
                                    "designed, not built", "n_gpus": args.gpus}))
         return 2
     # the CPU baseline's worker processes are forked before anything touches the GPU

@@ -188,7 +188,7 @@ int ctr_pixel_backproject(const ctr_geometry* g, const double* angles, const flo
                           size_t workspace_bytes, void* stream);
 
 /* ---- Learned prior: 3x3 convolutions, bfloat16 on MFMA, FP32 accumulation ---------------
- * No counterpart in the reference (the learned residual prior is new in this build).
+ * No counterpart in the reference (the reference has no learned prior).
  * Activations are NHWC bfloat16 [n][h][w][c]; weights bfloat16 [cout][3][3][cin]; bias float32.
  * ctr_conv3x3_bf16_forward: out = act(conv(in, weight) + bias) + residual ("same" zero
  * padding; act = ReLU if relu != 0; bias and residual may be NULL); cout <= 64.
