@@ -96,6 +96,23 @@ def test_forward_matches_oracle(dev, name):
     assert err <= TOL
 
 
+def test_forward_into_unaligned_output_view(dev):
+    """An output view 4 bytes off a 16-byte boundary (nu a multiple of 4): the row stores fall
+    back from 16-byte to per-column writes, with the same values, and nothing outside the view
+    is written."""
+    p = make_params(**CASES["flat"])
+    geom = ctr.ScanGeometry(p)
+    vol = torch.from_numpy(_vol(O.Geometry(p), 11)).to(dev)
+    ref = ctr.dd_forward(vol, geom)
+    n = ref.numel()
+    buf = torch.full((n + 2,), -7.0, dtype=torch.float32, device=dev)
+    view = buf[1:n + 1].view(ref.shape)
+    assert view.data_ptr() % 16 == 4
+    ctr.dd_forward(vol, geom, out=view)
+    torch.testing.assert_close(view, ref, rtol=0, atol=0)
+    assert float(buf[0]) == -7.0 and float(buf[-1]) == -7.0
+
+
 @pytest.mark.parametrize("name", sorted(CASES))
 @pytest.mark.parametrize("mode", ["ref", "edge"])
 def test_backproject_matches_oracle(dev, name, mode):
