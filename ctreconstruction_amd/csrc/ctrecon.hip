@@ -990,7 +990,8 @@ __device__ inline int window_mode(int wvn, int wun) {
 
 // Work split of the backprojection (kind): the fast pass (kBpFast) handles the common case
 // only -- LDS window, z chunk on one side of the source (affine footprints), at most kMaxRows
-// rows and kMaxUTaps columns, no weight zeroing -- and reports every (work-item, view) it
+// rows and kMaxUTaps columns, slowly varying footprint heights, no weight zeroing -- and
+// reports every (work-item, view) it
 // leaves; the completing pass (kBpRest) does exactly those; kBpAll does everything.  Keeping
 // the rare cases out of the fast pass keeps its register count, and so its occupancy, low.
 constexpr int kBpFast = 0, kBpAll = 1, kBpRest = 2;
@@ -1105,11 +1106,22 @@ __device__ inline void bp_row_integral3(const float* __restrict__ rp, int stride
 // window contribute nothing).  Without it the footprints of all kBpSub voxels must lie inside
 // the table's rows (bp_sub_rows' `inside`: the sub-chunk is whole and its rows were not cut by
 // the detector or the window); the clamp then never binds, so both ways give the same sums.
-// 1/(hi - lo) of voxel i comes by one Newton step from voxel i-1's: the footprint height is
-// affine in i and changes by a small fraction d per voxel (the magnification spread over one
-// voxel), so the step leaves a relative error of about d^2 (below 1e-8 at 512^3) and costs two
-// FMAs instead of a reciprocal, which issues at half the FMA rate.
-template <int OFF, bool CLAMP, int ZC>
+// Without EXACT, 1/(hi - lo) of voxel i comes by one Newton step from voxel i-1's: the
+// footprint height is affine in i and changes by a fraction d per voxel (the magnification
+// spread over one voxel), so the step leaves a relative error of about d^2 and costs two FMAs
+// instead of a reciprocal, which issues at half the FMA rate.  It is used only where
+// bp_newton_ok holds for the voxel column (d <= kNewtonRel, error below 4e-6; d is ~3.5e-4 at
+// 1000 mm from the source); with EXACT every voxel takes its own reciprocal.
+constexpr float kNewtonRel = 2e-3f;
+
+// The per-voxel change d of the footprint height of any affine sub-chunk of a column: with
+// h_0 = |Ahi - Alo| >= |Bv| hz (rmax + rmin) and |Dhi - Dlo| = |Bv| 2 hz (rmax - rmin) on either
+// side of the source height, d <= 2 (rmax - rmin) / (rmax + rmin), a bound of the column alone.
+__device__ inline bool bp_newton_ok(const BpColumn& col) {
+    return 2.0f * (col.rmax - col.rmin) <= kNewtonRel * (col.rmax + col.rmin);
+}
+
+template <int OFF, bool CLAMP, bool EXACT, int ZC>
 __device__ inline void bp_voxels_affine(const BpChunk& ch, int ks, int nr, const float* __restrict__ scan,
                                         float (&acc)[ZC]) {
     static_assert(OFF + kBpSub <= ZC, "sub-chunk inside the work-item");
@@ -1119,7 +1131,10 @@ __device__ inline void bp_voxels_affine(const BpChunk& ch, int ks, int nr, const
 #pragma unroll
     for (int i = 0; i < kBpSub; ++i) {
         const float lo = fmaf((float)i, ch.Dlo, alo), hi = fmaf((float)i, ch.Dhi, ahi);
-        if (i > 0) rinv *= fmaf(lo - hi, rinv, 2.0f);
+        if (EXACT)
+            rinv = __builtin_amdgcn_rcpf(hi - lo);
+        else if (i > 0)
+            rinv *= fmaf(lo - hi, rinv, 2.0f);
         const float xl = CLAMP ? __builtin_amdgcn_fmed3f(lo, 1.0f, top) : lo;
         const float xh = CLAMP ? __builtin_amdgcn_fmed3f(hi, 1.0f, top) : hi;
         acc[OFF + i] = fmaf(bp_lookup(scan, xh) - bp_lookup(scan, xl), rinv, acc[OFF + i]);
@@ -1155,8 +1170,9 @@ __device__ inline void bp_for_subs(F&& f) {
 }
 
 // The fast pass (kBpFast): the common case only -- window in LDS, sub-chunk on one side of the
-// source height, at most kMaxRows rows and kMaxUTaps columns, no weight zeroing (the host runs
-// this pass only with zero_tol = 0).  The column's u-footprint, u-weights and FDK weight are
+// source height, at most kMaxRows rows and kMaxUTaps columns, a footprint height that changes
+// slowly enough along z for the Newton-step normaliser (bp_newton_ok), no weight zeroing (the
+// host runs this pass only with zero_tol = 0).  The column's u-footprint, u-weights and FDK weight are
 // worked out once per view for all ZC voxels; every sub-chunk then builds its own row integral.
 // Returns true when it leaves a sub-chunk of this (work-item, view) to the completing pass,
 // which decides from the same bp_column / bp_chunk values.
@@ -1183,6 +1199,7 @@ __device__ inline bool bp_column_view_fast(const BpView& v, const float* __restr
         wu[j] = (kus + j <= kue) ? fmaxf(w, 0.0f) : 0.0f;
     }
     if (col.ku0 == col.ku1) wu[0] = 1.0f;
+    const bool newton_ok = bp_newton_ok(col);
     const float dd = v.R - col.yr0;
     const float W = scale * (v.R * v.R) * __builtin_amdgcn_rcpf(dd * dd);
     const float* wp = win + (kus - col0);
@@ -1200,13 +1217,13 @@ __device__ inline bool bp_column_view_fast(const BpView& v, const float* __restr
         bool inside;
         bp_sub_rows(ch, n, row0, nrows, nv, ks, nr, inside);
         if (nr <= 0) return;
-        if (nr > kMaxRows || !taps_ok || !inside) {
+        if (nr > kMaxRows || !taps_ok || !inside || !newton_ok) {
             left = true;
             return;
         }
         // window offsets fit 24 bits: a full-rate multiply (the 32-bit one is quarter rate)
         bp_row_integral3(wp + __mul24(ks - row0, stride), stride, nr, wu, W, scan);
-        bp_voxels_affine<kBpSub * h, false>(ch, ks, nr, scan, acc);
+        bp_voxels_affine<kBpSub * h, false, false>(ch, ks, nr, scan, acc);
     });
     return left;
 }
@@ -1277,8 +1294,8 @@ __device__ inline void bp_column_view(const BpView& v, const float* __restrict__
         if (nr <= 0) return;
         if constexpr (KIND == kBpRest) {
             // the sub-chunks the fast pass did
-            const bool common =
-                window_in_lds && lin && inside && nr <= kMaxRows && ntaps <= kMaxUTaps && zero_tol <= 0.0f;
+            const bool common = window_in_lds && lin && inside && nr <= kMaxRows && ntaps <= kMaxUTaps &&
+                                zero_tol <= 0.0f && bp_newton_ok(col);
             if (common) return;
         }
         if (zero_tol <= 0.0f && nr <= kMaxRows) {
@@ -1302,7 +1319,7 @@ __device__ inline void bp_column_view(const BpView& v, const float* __restrict__
                 scan[(nr + 2) * 256] = csum;
             }
             if (lin) {
-                bp_voxels_affine<OFF, true>(ch, ks, nr, scan, acc);
+                bp_voxels_affine<OFF, true, true>(ch, ks, nr, scan, acc);
             } else {
                 const float base = (float)(ks - 1), top = (float)(nr + 1);
 #pragma unroll

@@ -49,6 +49,13 @@ CASES = {
     # ... more than 166 cells: the forward projector's explicit per-line path
     "coarse_rows": dict(NumberOfImage=[12, 12, 256], NumberOfDetectorPixels=[24, 72],
                         DetectorPixelSize=[1.2, 2.4], NumberOfViews=6),
+    # the source 60 mm from the isocentre: a voxel column's footprint height changes by ~2.4 %
+    # per voxel along z, too fast for the backprojector's Newton-step normaliser, so its
+    # sub-chunks take exact reciprocals in the completing pass
+    "near_source": dict(SourceInit=[0, 60.0, 0], DetectorInit=[0, -40.0, 0], NumberOfImage=[16, 16, 32],
+                        ImagePixelSpacing=[1.0, 1.0, 1.0], NumberOfDetectorPixels=[20, 40],
+                        DetectorPixelSize=[2.0, 2.0], NumberOfViews=12, StartAngle=0.1,
+                        EndAngle=0.1 + 2 * np.pi),
 }
 
 
@@ -441,7 +448,7 @@ def test_split_cases_reach_the_completing_pass():
     assert (8 * mag_lo / du + 2) * rows < 1024 < (8 * np.sqrt(2) * mag_hi / du + 2) * rows
 
 
-@pytest.mark.parametrize("name", sorted(SPLIT_CASES) + ["flat"])
+@pytest.mark.parametrize("name", sorted(SPLIT_CASES) + ["flat", "near_source"])
 @pytest.mark.parametrize("mode", ["ref", "edge"])
 def test_two_pass_backprojection_equals_one_pass(dev, name, mode):
     """The common-case pass plus the completing pass add every (voxel, view) pair exactly once:
