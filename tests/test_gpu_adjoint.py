@@ -39,7 +39,7 @@ def _rel(a, b):
 
 
 @pytest.mark.parametrize("name", ["flat", "flat_45deg_ties", "flat_offsets", "anisotropic", "helical",
-                                  "curved", "wide_footprint", "curved_fan_2d", "near_source"])
+                                  "curved", "wide_footprint", "curved_fan_2d", "near_source", "off_center"])
 def test_adjoint_matches_oracle_transpose(dev, name):
     p = make_params(**CASES[name])
     g = O.Geometry(p)

@@ -49,6 +49,9 @@ CASES = {
     # ... more than 166 cells: the forward projector's explicit per-line path
     "coarse_rows": dict(NumberOfImage=[12, 12, 256], NumberOfDetectorPixels=[24, 72],
                         DetectorPixelSize=[1.2, 2.4], NumberOfViews=6),
+    # a volume far off the rotation axis (shifted by about half its width in x, y and z)
+    "off_center": dict(NumberOfImage=[24, 20, 18], ReconCenter=[6.0, -5.0, 4.0], NumberOfDetectorPixels=[48, 40],
+                       NumberOfViews=12, StartAngle=0.2, EndAngle=0.2 + 2 * np.pi),
     # the source 60 mm from the isocentre: a voxel column's footprint height changes by ~2.4 %
     # per voxel along z, too fast for the backprojector's Newton-step normaliser, so its
     # sub-chunks take exact reciprocals in the completing pass
