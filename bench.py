@@ -10,10 +10,9 @@ forward projection is timed separately and reported beside the step.
 
 Prints one JSON line.  ``value`` is seconds per FDK reconstruction (lower is better);
 ``roofline`` prices the dominant kernel (the backprojection) against the 8 TB/s HBM peak
-with algorithmic bytes 4*(views*nv*nu + nx*ny*nz) per launch; ``cpu_baseline`` times the FP64
-NumPy oracle on a bounded sample spread over one process per host core (two views per process,
-central 16-slice slab) and scales the wall time to a full reconstruction; a single-thread run
-(2 views, central 8-slice slab) is reported beside it.
+with algorithmic bytes 4*(views*nv*nu + nx*ny*nz) per launch; ``cpu_baseline`` times the oracle's C
+restatement (FP64, OpenMP on every host core) on 8 views spread over the scan into the whole
+volume and scales the wall time to 720 views; a one-thread run of one view is reported beside it.
 
 ``pixel_fdk`` times the reference's other FDK, the pixel-driven one (voxel-driven bilinear
 backprojection, python/ConeBeamGPU.py:203-321), on the reference's geometry file
@@ -84,109 +83,63 @@ def profile_record(name, kernel):
     return rec if rec.get("kernel") == kernel else {}
 
 
-def _oracle_views(args):
-    """Worker of the multi-core CPU baseline: FP64 oracle filter + FDK backprojection of some
-    views into the central slab; returns (slab volume, filter seconds, backprojection seconds)."""
-    params, views, raw, slab = args
-    from oracle import dd_oracle as O
+def cpu_baseline(params, raw, geom, sample=8, check=4):
+    """Time the CPU port of the FDK step and scale it to a full reconstruction.
 
-    g = O.Geometry(params)
-    t0 = time.perf_counter()
-    q = O.filter_projections(g, raw)
-    t_f = time.perf_counter() - t0
-    p_slab = dict(params)
-    p_slab["NumberOfImage"] = [g.nx, g.ny, slab]
-    gs = O.Geometry(p_slab)
-    t0 = time.perf_counter()
-    vol = np.zeros((slab, g.ny, g.nx))
-    for n, i in enumerate(views):
-        vol += O.backproject_view(gs, q[n].astype(np.float64) * g.dtheta, i, "ref")
-    return vol, t_f, time.perf_counter() - t0
-
-
-def _warm(_):
-    from oracle import dd_oracle  # noqa: F401
-    return os.getpid()
-
-
-def cpu_baseline(params, q_gpu, geom, dev, pool=None, sample_views=(0, 1), slab=8):
-    """Time the FP64 NumPy oracle (filter + FDK backprojection) on a bounded sample and scale
-    it to a full reconstruction; also return the sample's error against the GPU result.  With a
-    process pool the sample (two views per worker, 16-slice central slab) is spread over the
-    host's cores and the wall time is scaled; the single-thread figure is reported beside it."""
-    import torch
-
+    The port is the oracle's C restatement (``oracle/c/dd_backproject.c``: FP64, the oracle's
+    operations in its order, OpenMP threads over the image rows) with the oracle's filter batched
+    over views on ``scipy.fft`` workers, both on every host core this process may use.  Sample:
+    filter + backprojection of ``sample`` views spread evenly over the scan into the whole volume;
+    the wall time is scaled by nview / sample (each view of a circular scan is the same work).
+    ``check`` views are timed as well to show that the per-view time holds, and one view on one
+    thread gives the single-core figure.  The sample's volume and filtered rows are compared with
+    the GPU's filter and backprojection of the same views."""
     import ctreconstruction_amd as ctr
+    from oracle import dd_c as C
     from oracle import dd_oracle as O
 
     g = O.Geometry(params)
-    slab = min(slab, g.nz)
-    views = list(sample_views)
-    # filter: time the oracle on the sample views
-    proj_s = q_gpu["raw"][views].cpu().numpy()
-    t0 = time.perf_counter()
-    q_ref = O.filter_projections(g, proj_s)
-    t_filter = (time.perf_counter() - t0) / len(views)
-    # backprojection: central slab of `slab` slices, sample views
-    p_slab = dict(params)
-    p_slab["NumberOfImage"] = [g.nx, g.ny, slab]
-    gs = O.Geometry(p_slab)
-    t0 = time.perf_counter()
-    vol_ref = np.zeros((slab, g.ny, g.nx))
-    for n, i in enumerate(views):
-        vol_ref += O.backproject_view(gs, q_ref[n].astype(np.float64) * g.dtheta, i, "ref")
-    t_bp = (time.perf_counter() - t0) / len(views)
-    # the same sample on the GPU: filter of the sample views, backprojection into the slab
-    geom_slab = ctr.ScanGeometry(p_slab).with_views(views)
-    q_s = ctr.fdk_filter(q_gpu["raw"][views].contiguous(), geom)
-    vol_gpu = ctr.dd_backproject(q_s, geom_slab).cpu().numpy()
-    err = float(np.abs(vol_gpu - vol_ref).max() / np.abs(vol_ref).max())
-    err_filter = float(np.abs(q_s.cpu().numpy() - q_ref).max() / np.abs(q_ref).max())
-    single = t_filter * g.nview + t_bp * g.nview * (g.nz / slab)
-    out = {
-        "value": single, "unit": "s", "cores": 1, "kind": "port",
-        "sample": (f"FP64 NumPy oracle, single thread: filter of {len(views)} views and FDK "
-                   f"backprojection of {len(views)} views into the central {slab}-slice slab, "
-                   f"scaled linearly to {g.nview} views x {g.nz} slices "
-                   f"(filter {t_filter:.3f} s/view, backprojection {t_bp:.3f} s/view/slab)"),
-        "host_cpus": os.cpu_count(), "host_affinity": len(os.sched_getaffinity(0)),
-        "sample_error_vs_oracle": {"fdk_backprojection": err, "filter": err_filter},
+    threads = C.host_threads()
+    C.lib()
+
+    def spread(k):
+        return [int(round(j * g.nview / k)) for j in range(k)]
+
+    def timed(views, t):
+        r = raw[views].cpu().numpy()
+        t0 = time.perf_counter()
+        q = C.filter_projections(g, r, threads=t)
+        t1 = time.perf_counter()
+        vol = C.backproject(g, q, "ref", views=views, threads=t)
+        t2 = time.perf_counter()
+        return q, vol, t1 - t0, t2 - t1
+
+    views = spread(sample)
+    q_c, vol_c, t_f, t_bp = timed(views, threads)
+    full = (t_f + t_bp) * g.nview / sample
+    _, _, t_f_chk, t_bp_chk = timed(spread(check), threads)
+    full_chk = (t_f_chk + t_bp_chk) * g.nview / check
+    _, _, t_f_1, t_bp_1 = timed(views[:1], 1)
+    single = (t_f_1 + t_bp_1) * g.nview
+    q_s = ctr.fdk_filter(raw[views].contiguous(), geom)
+    vol_gpu = ctr.dd_backproject(q_s, geom.with_views(views)).cpu().numpy()
+    err = float(np.abs(vol_gpu - vol_c).max() / np.abs(vol_c).max())
+    err_filter = float(np.abs(q_s.cpu().numpy() - q_c).max() / np.abs(q_c).max())
+    return {
+        "value": full, "unit": "s", "cores": threads, "kind": "port",
+        "sample": (f"C restatement of the FP64 oracle (gcc -O3, OpenMP) on {threads} threads: filter "
+                   f"and FDK backprojection of {sample} views spread over the scan into the whole "
+                   f"{g.nx}x{g.ny}x{g.nz} volume in {t_f + t_bp:.2f} s wall (filter {t_f:.2f} s, "
+                   f"backprojection {t_bp:.2f} s), scaled linearly to {g.nview} views"),
+        "cpu_seconds_in_sample": (t_f + t_bp) * threads,
+        "extrapolation_check": {f"{check}_views_s": full_chk, f"{sample}_views_s": full,
+                                "ratio": full / full_chk},
         "single_thread_s": single,
+        "single_thread_sample": (f"1 view filtered and backprojected on 1 thread in {t_f_1 + t_bp_1:.2f} s "
+                                 f"(filter {t_f_1:.3f} s), scaled to {g.nview} views"),
+        "host_cpus": os.cpu_count(), "host_affinity": threads,
+        "sample_error_vs_oracle": {"fdk_backprojection": err, "filter": err_filter},
     }
-    if pool is not None:
-        workers = pool._processes
-        mslab = min(16, g.nz)
-        raw = q_gpu["raw"]
-
-        def run(per_worker):
-            # per_worker views per process, spread over the scan
-            step = max(g.nview // (per_worker * workers), 1)
-            sample = [list(range(w * per_worker * step, (w + 1) * per_worker * step, step)) for w in range(workers)]
-            jobs = [(params, v, raw[v].cpu().numpy(), mslab) for v in sample]
-            t0 = time.perf_counter()
-            res = pool.map(_oracle_views, jobs)
-            wall = time.perf_counter() - t0
-            n_done = sum(len(v) for v in sample)
-            return res, wall, n_done, wall * (g.nview / n_done) * (g.nz / mslab)
-
-        # linearity of the extrapolation: one view per process, then two
-        _, wall1, n1, full1 = run(1)
-        res, wall, n_done, full = run(2)
-        out["extrapolation_check"] = {
-            "one_view_per_process_s": full1, "two_views_per_process_s": full,
-            "ratio": full / full1,
-            "note": (f"{n1} views in {wall1:.2f} s wall and {n_done} views in {wall:.2f} s wall, each "
-                     f"scaled linearly to the full reconstruction")}
-        out.update({
-            "value": full, "cores": workers,
-            "sample": (f"FP64 NumPy oracle on {workers} processes (one per host core): filter and "
-                       f"FDK backprojection of {n_done} views (two per process, spread over the "
-                       f"scan) into the central {mslab}-slice slab in {wall:.2f} s wall, scaled "
-                       f"linearly to {g.nview} views x {g.nz} slices; the single-thread run of "
-                       f"2 views x {slab} slices scales to {single:.0f} s"),
-            "cpu_seconds_in_sample": float(sum(r[1] + r[2] for r in res)),
-        })
-    return out
 
 
 WEIGHTS = os.path.join(ROOT, "ctreconstruction_amd", "weights", "prior_sparse60.safetensors")
@@ -356,13 +309,6 @@ def main(argv=None):
         print(json.dumps({"error": "bench.py runs on one GPU; the view-sharded multi-GPU layout is "
                                    "designed, not built", "n_gpus": args.gpus}))
         return 2
-    # the CPU baseline's worker processes are forked before anything touches the GPU
-    pool = None
-    if not args.no_cpu_baseline:
-        import multiprocessing as mp
-        pool = mp.get_context("fork").Pool(len(os.sched_getaffinity(0)))
-        pool.map(_warm, range(pool._processes))
-
     import torch
 
     import ctreconstruction_amd as ctr
@@ -477,10 +423,7 @@ def main(argv=None):
     if calib is not None:
         out["valu_calibration"] = calib
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(params, {"raw": proj}, geom, dev, pool)
-    if pool is not None:
-        pool.close()
-        pool.join()
+        out["cpu_baseline"] = cpu_baseline(params, proj, geom)
     print(json.dumps(out))
     return 0
 
