@@ -1,5 +1,6 @@
 """ctypes front end of the C restatement of the oracle's FDK backprojection (``c/dd_backproject.c``)
-and a multi-threaded restatement of its filter.
+and a multi-threaded restatement of its filter; the library also holds the pixel oracle's
+backprojection (``c/pixel_backproject.c``, front end in :mod:`oracle.pixel_c`).
 
 TEST INFRASTRUCTURE: used by the tests, the smoke check and the benchmark's CPU-baseline leg as
 the checker and as the CPU time of a compiled port; the product path never imports it.
@@ -21,19 +22,19 @@ import numpy as np
 from . import dd_oracle as O
 
 ORACLE_DIR = os.path.dirname(os.path.abspath(__file__))
-SOURCE = os.path.join(ORACLE_DIR, "c", "dd_backproject.c")
-LIB_PATH = os.path.join(ORACLE_DIR, "lib", "libddoracle.so")
+SOURCES = [os.path.join(ORACLE_DIR, "c", "dd_backproject.c"), os.path.join(ORACLE_DIR, "c", "pixel_backproject.c")]
+LIB_PATH = os.path.join(ORACLE_DIR, "lib", "liboracle.so")
 CFLAGS = ["-O3", "-fopenmp", "-ffp-contract=off", "-fPIC", "-shared", "-std=c11", "-Wall", "-Wextra"]
 
 _lib = None
 
 
 def build() -> str:
-    """Compile ``c/dd_backproject.c`` into ``lib/libddoracle.so`` with gcc (always recompiles)."""
+    """Compile ``c/*.c`` into ``lib/liboracle.so`` with gcc (always recompiles)."""
     global _lib
     os.makedirs(os.path.dirname(LIB_PATH), exist_ok=True)
     tmp = LIB_PATH + ".tmp"
-    subprocess.run(["gcc", *CFLAGS, SOURCE, "-o", tmp, "-lm"], check=True)
+    subprocess.run(["gcc", *CFLAGS, *SOURCES, "-o", tmp, "-lm"], check=True)
     os.replace(tmp, LIB_PATH)
     _lib = None
     return LIB_PATH
@@ -54,6 +55,10 @@ def lib():
             ctypes.c_double, dp, dp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
             dp, dp, dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
             ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, dp]
+        L.pxo_backproject.restype = ctypes.c_int
+        L.pxo_backproject.argtypes = [
+            dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp, ctypes.c_double, ctypes.c_double,
+            dp, dp, dp, dp, dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]
         _lib = L
     return _lib
 

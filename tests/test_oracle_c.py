@@ -109,3 +109,69 @@ def test_planted_mistake_shows(monkeypatch):
 
     monkeypatch.setattr(O, "_overlap_weights", swapped)
     assert np.abs(O.backproject(g, q, "ref") - got_b).max() > 1e-3 * np.abs(got_b).max()
+
+
+# ---------------------------------------------------------------- pixel-driven backprojection
+
+from oracle import pixel_c as PC  # noqa: E402
+from oracle import pixel_oracle as PO  # noqa: E402
+from tests.conftest import make_cone_params  # noqa: E402
+
+PIXEL_CASES = {
+    "default": {},
+    # odd detector and volume: the central column and slice sample exactly on a node at 0 degrees
+    "nodes": {"DetectorWidth": 41, "DetectorHeight": 33, "ReconX": 25, "ReconY": 24, "ReconZ": 17,
+              "NumberOfViews": 12},
+    "short_scan": {"AngleCoverage": 1.2 * np.pi, "NumberOfViews": 20, "ReconZ": 9},
+    # a volume wider than the detector's view: samples off the grid
+    "off_grid": {"DetectorWidth": 20, "DetectorHeight": 12, "NumberOfViews": 10},
+}
+
+
+def _pixel_setup(name, seed=0):
+    g = PO.PixelGeometry(make_cone_params(**PIXEL_CASES[name]))
+    if name == "off_grid":
+        g.x = g.x * 1.6
+        g.z = g.z * 1.8
+    proj = np.random.default_rng(seed).uniform(0.0, 2.0, (g.ns, g.nv, g.nu))
+    return g, proj
+
+
+@pytest.mark.parametrize("name", sorted(PIXEL_CASES))
+def test_c_pixel_backprojection_and_margin_equal_numpy_oracle(name):
+    g, proj = _pixel_setup(name)
+    q = PO.filter_projections(g, proj)
+    views = list(range(g.ns))
+    ref = PO.backproject(g, q)
+    got, margin = PC.backproject(g, q, views, threads=3, margin=True)
+    assert np.abs(ref).max() > 0
+    assert np.array_equal(got, ref)
+    assert np.array_equal(margin, PO.edge_margin(g))
+    if name == "off_grid":
+        assert (margin > 0).all() and (ref == 0).any()
+
+
+def test_c_pixel_backprojection_view_subset_and_scale():
+    g, proj = _pixel_setup("short_scan", seed=1)
+    q = PO.filter_projections(g, proj)
+    views = [2, 9, 15]
+    ref = PO.backproject(g, {i: q[i] for i in views}, scale=0.5, views=views)
+    assert np.array_equal(PC.backproject(g, q[views], views, scale=0.5, threads=1), ref)
+
+
+def test_c_pixel_planted_mistake_shows(monkeypatch):
+    """Interpolating on nodes shifted by half a cell in the NumPy loop must break the equality."""
+    g, proj = _pixel_setup("default", seed=2)
+    q = PO.filter_projections(g, proj)
+    got = PC.backproject(g, q, range(g.ns), threads=2)
+    orig = PO.interpolate
+    monkeypatch.setattr(PO, "interpolate", lambda q, ki, p, U, V: orig(q, ki + (ki[1] - ki[0]) / 2, p, U, V))
+    assert np.abs(PO.backproject(g, q) - got).max() > 1e-3 * np.abs(got).max()
+
+
+@pytest.mark.parametrize("name", ["default", "nodes"])
+def test_batched_pixel_filter_equals_numpy_oracle(name):
+    g, proj = _pixel_setup(name, seed=3)
+    ref = PO.filter_projections(g, proj)
+    got = PC.filter_projections(g, proj, threads=3, chunk=5)
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
