@@ -307,16 +307,16 @@ def learned_vs_sart(seeds=HELD_OUT_SEEDS):
     if not os.path.exists(WEIGHTS):
         return None
     from ctreconstruction_amd import learned as L
-    net = L.load_weights(WEIGHTS)
-    factory = L.PairFactory()
+    factory, net, dc_steps = L.pipeline_for(WEIGHTS)
     per_seed = []
     for seed in seeds:
-        res = L.time_to_rmse(factory, net, seed)
+        res = L.time_to_rmse(factory, net, seed, dc_steps=dc_steps)
         res["phantom_seed"] = seed
         per_seed.append(res)
     sp = [r["speedup"] for r in per_seed]
     spr = [r["speedup_to_rmse"] for r in per_seed]
     return {"phantom_seeds": list(seeds),
+            "pipeline": f"SART-{factory.sart_iters} warm start, network, {dc_steps} SART data-consistency steps",
             "speedup_min": min(sp), "speedup_mean": float(np.mean(sp)),
             "speedup_to_rmse_min": min(spr), "speedup_to_rmse_mean": float(np.mean(spr)),
             "learned_reaches_plain_rmse_all": all(r["learned_reaches_plain_rmse"] for r in per_seed),

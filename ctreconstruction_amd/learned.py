@@ -4,12 +4,13 @@ Scan (the sparse-view benchmark): 256^3 volume at 0.5 mm of seeded random ellips
 mu_water = 0.02/mm, 384 x 384 flat detector at 1.0 mm, SID 1000, SDD 1500, Poisson noise at
 1e5 photons.  A training pair is made by the method itself from one phantom:
 
-* input  = 20-iteration SART (relaxation 0.5, matched backprojector) of the 60-view noisy scan,
+* input  = 12-iteration SART (relaxation 0.5, matched backprojector) of the 60-view noisy scan,
 * target = FDK of the 720-view noisy scan (Shepp-Logan window).
 
 The network (``prior.ResidualPrior``) sees 3 adjacent slices of the input and predicts the
-target's centre slice.  At inference the learned reconstruction is: 20 SART iterations, the
-network on every slice, then 5 SART data-consistency iterations starting from its output.
+target's centre slice.  At inference the learned reconstruction is: 12 SART iterations, the
+network on every slice, then 5 SART data-consistency iterations starting from its output (the
+weights' metadata records both counts; ``pipeline_for`` reads them).
 Plain SART is the fallback and the arbiter: if the learned result fits the measured projections
 worse than the SART iterate it started from (projection-data residual), plain SART continues
 instead.  Reconstructions are in mm^-1; the network works in units of mu_water (x 50).
@@ -44,7 +45,7 @@ class PairFactory:
     """Seeded (input, target, phantom) volumes for the sparse-view benchmark, on the GPU."""
 
     def __init__(self, device="cuda", n: int = 256, sparse_views: int = 60, full_views: int = 720,
-                 sart_iters: int = 20, relaxation: float = 0.5, photons: float = 1e5,
+                 sart_iters: int = 12, relaxation: float = 0.5, photons: float = 1e5,
                  n_ellipsoids: int = 20, backprojector: str = "matched"):
         self.device = torch.device(device)
         self.sparse = ScanGeometry(scan_params(sparse_views, n))
@@ -143,7 +144,8 @@ def learned_reconstruction(factory: PairFactory, net: PR.ResidualPrior, proj: to
     return x, info
 
 
-def time_to_rmse(factory: PairFactory, net: PR.ResidualPrior, seed: int, plain_iters: int = 200):
+def time_to_rmse(factory: PairFactory, net: PR.ResidualPrior, seed: int, plain_iters: int = 200,
+                 dc_steps: int = 5):
     """Wall time of plain SART (``plain_iters`` iterations) against the learned pipeline, and
     the RMSE (vs the phantom) of both.
 
@@ -165,10 +167,10 @@ def time_to_rmse(factory: PairFactory, net: PR.ResidualPrior, seed: int, plain_i
     IT.sart(proj, g, plain_iters, relaxation=factory.relaxation, backprojector=factory.backprojector)
     torch.cuda.synchronize()
     t_plain = time.perf_counter() - t0
-    learned_reconstruction(factory, net, proj)  # warm-up: kernel loading is not timed
+    learned_reconstruction(factory, net, proj, dc_steps)  # warm-up: kernel loading is not timed
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    x_l, info = learned_reconstruction(factory, net, proj)
+    x_l, info = learned_reconstruction(factory, net, proj, dc_steps)
     torch.cuda.synchronize()
     t_learned = time.perf_counter() - t0
     r_l = rmse(x_l)
@@ -186,6 +188,23 @@ def save_weights(net: PR.ResidualPrior, path: str, meta: dict | None = None) -> 
     from safetensors.torch import save_file
     save_file({k: v.detach().float().cpu().contiguous() for k, v in net.state_dict().items()}, path,
               metadata={k: str(v) for k, v in (meta or {}).items()})
+
+
+def weights_metadata(path: str) -> dict:
+    """The training record stored with the weights (string values)."""
+    from safetensors import safe_open
+    with safe_open(path, framework="pt") as f:
+        return dict(f.metadata() or {})
+
+
+def pipeline_for(path: str, device="cuda"):
+    """(factory, net, dc_steps) of the pipeline the weights at ``path`` were trained for: the
+    warm start's SART iteration count and the data-consistency steps come from their metadata
+    (20 and 5 when absent)."""
+    meta = weights_metadata(path)
+    factory = PairFactory(device=device, sart_iters=int(meta.get("sart_iters", 20)),
+                          backprojector=meta.get("backprojector", "matched"))
+    return factory, load_weights(path, device), int(meta.get("dc_steps", 5))
 
 
 def load_weights(path: str, device="cuda") -> PR.ResidualPrior:
@@ -228,8 +247,10 @@ def main(argv=None):
     ap.add_argument("--eval-seed", type=int, default=10000)
     ap.add_argument("--no-eval", action="store_true")
     ap.add_argument("--backprojector", default="matched", choices=["ref", "edge", "matched"])
+    ap.add_argument("--sart-iters", type=int, default=12, help="SART iterations of the warm start")
+    ap.add_argument("--dc-steps", type=int, default=5, help="SART data-consistency steps after the network")
     args = ap.parse_args(argv)
-    factory = PairFactory(backprojector=args.backprojector)
+    factory = PairFactory(backprojector=args.backprojector, sart_iters=args.sart_iters)
     if args.resume:
         net, opt, start = load_checkpoint(args.resume, args.lr)
     else:
@@ -244,10 +265,11 @@ def main(argv=None):
         save_checkpoint(net, opt, end, args.checkpoint)
     save_weights(net, args.out, {"pairs": end, "steps_per_pair": args.steps_per_pair,
                                  "batch": args.batch, "crop": args.crop, "lr": args.lr,
-                                 "backprojector": args.backprojector})
+                                 "backprojector": args.backprojector, "sart_iters": args.sart_iters,
+                                 "dc_steps": args.dc_steps})
     print(json.dumps({"trained_pairs": end, "train_s_this_run": t_train}), flush=True)
     if not args.no_eval:
-        res = time_to_rmse(factory, net, args.eval_seed)
+        res = time_to_rmse(factory, net, args.eval_seed, dc_steps=args.dc_steps)
         print(json.dumps(res), flush=True)
 
 
