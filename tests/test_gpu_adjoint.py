@@ -39,7 +39,9 @@ def _rel(a, b):
 
 
 @pytest.mark.parametrize("name", ["flat", "flat_45deg_ties", "flat_offsets", "anisotropic", "helical",
-                                  "curved", "wide_footprint", "curved_fan_2d", "near_source", "off_center"])
+                                  "curved", "wide_footprint", "curved_fan_2d", "near_source", "off_center",
+                                  "reverse", "helical_reverse", "short_scan", "source_z", "origin",
+                                  "single_view", "thin_z"])
 def test_adjoint_matches_oracle_transpose(dev, name):
     p = make_params(**CASES[name])
     g = O.Geometry(p)

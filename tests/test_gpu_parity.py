@@ -59,6 +59,25 @@ CASES = {
                         ImagePixelSpacing=[1.0, 1.0, 1.0], NumberOfDetectorPixels=[20, 40],
                         DetectorPixelSize=[2.0, 2.0], NumberOfViews=12, StartAngle=0.1,
                         EndAngle=0.1 + 2 * np.pi),
+    # rotation in the negative sense: decreasing angles, dtheta < 0 (the reference's linspace
+    # allows it, and Q = proj * dtheta changes sign with it)
+    "reverse": dict(NumberOfImage=[24, 24, 20], NumberOfDetectorPixels=[40, 32], NumberOfViews=12,
+                    StartAngle=0.1 + 2 * np.pi, EndAngle=0.1),
+    "helical_reverse": dict(NumberOfImage=[20, 20, 24], NumberOfDetectorPixels=[36, 20], NumberOfViews=16,
+                            Pitch=0.5, StartAngle=4 * np.pi, EndAngle=0.0),
+    "short_scan": dict(NumberOfImage=[24, 22, 18], NumberOfDetectorPixels=[40, 32], NumberOfViews=30,
+                       StartAngle=0.05, EndAngle=0.05 + np.pi + 0.6),
+    # source and detector centre at different heights: the central ray is tilted
+    "source_z": dict(SourceInit=[0, 1000.0, 6.0], DetectorInit=[0, -500.0, -3.0], NumberOfImage=[24, 24, 20],
+                     NumberOfDetectorPixels=[40, 36], NumberOfViews=12, StartAngle=0.1, EndAngle=0.1 + 2 * np.pi),
+    # SAD measured from a rotation origin off the source's axis (900 mm instead of 1000)
+    "origin": dict(RotationOrigin=[0, 100.0, 0], NumberOfImage=[24, 24, 20], NumberOfDetectorPixels=[40, 32],
+                   NumberOfViews=12, StartAngle=0.1, EndAngle=0.1 + 2 * np.pi),
+    # one view: dtheta is EndAngle - StartAngle
+    "single_view": dict(NumberOfImage=[24, 24, 20], NumberOfDetectorPixels=[40, 32], NumberOfViews=1,
+                        StartAngle=0.7, EndAngle=0.7 + 2 * np.pi),
+    "thin_z": dict(NumberOfImage=[24, 24, 2], NumberOfDetectorPixels=[40, 32], NumberOfViews=12,
+                   StartAngle=0.1, EndAngle=0.1 + 2 * np.pi),
 }
 
 
@@ -213,7 +232,7 @@ def test_fdk_filter_in_place_and_odd_row_count(dev):
     assert _rel(t.cpu().numpy(), O.filter_projections(g, proj)) <= TOL
 
 
-@pytest.mark.parametrize("name", ["flat", "curved"])
+@pytest.mark.parametrize("name", ["flat", "curved", "reverse", "short_scan", "source_z", "single_view"])
 def test_fdk_end_to_end_matches_oracle(dev, name):
     p = make_params(FilterType="shepp-logan", **CASES[name])
     g = O.Geometry(p)
