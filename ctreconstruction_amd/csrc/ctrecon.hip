@@ -1071,30 +1071,31 @@ __device__ inline float bp_lookup(const float* __restrict__ scan, float x) {
 
 // The row integral of nr window rows from `rp` (stride `stride`), u-filtered with three taps
 // (weight 0 past the footprint; the window buffers are zero-initialised and padded, so such
-// reads stay finite) and times W, into the table above.  Two rows per step.
-__device__ inline void bp_row_integral3(const float* __restrict__ rp, int stride, int nr, const float (&wu)[kMaxUTaps],
-                                        float W, float* __restrict__ scan) {
+// reads stay finite) and times W, into the table above.  The tap weights `wa` come with W
+// folded in (wa[t] = wu[t] * W), so a row is three FMAs into the running sum, taps in the order
+// 2, 1, 0 (every pass accumulates a row this way).  Two rows per step.
+__device__ inline void bp_row_integral3(const float* __restrict__ rp, int stride, int nr, const float (&wa)[kMaxUTaps],
+                                        float* __restrict__ scan) {
+    const float a0 = wa[0], a1 = wa[1], a2 = wa[2];
     float csum = 0.0f;
     scan[0] = 0.0f;
     scan[256] = 0.0f;
     float* sp = scan + 256;
-    int j = 0;
-    for (; j + 1 < nr; j += 2) {
+    // the loop runs on the row pointer (no row counter to keep per lane)
+    const float* const pair_end = rp + __mul24(nr & ~1, stride);
+    while (rp != pair_end) {
         const float* rq = rp + stride;
         const float p0 = rp[0], p1 = rp[1], p2 = rp[2];
         const float q0 = rq[0], q1 = rq[1], q2 = rq[2];
-        const float rk0 = fmaf(wu[0], p0, fmaf(wu[1], p1, wu[2] * p2));
-        const float rk1 = fmaf(wu[0], q0, fmaf(wu[1], q1, wu[2] * q2));
-        csum = fmaf(rk0, W, csum);
+        csum = fmaf(a0, p0, fmaf(a1, p1, fmaf(a2, p2, csum)));
         sp[256] = csum;
-        csum = fmaf(rk1, W, csum);
+        csum = fmaf(a0, q0, fmaf(a1, q1, fmaf(a2, q2, csum)));
         sp[512] = csum;
         sp += 512;
         rp += 2 * stride;
     }
-    if (j < nr) {
-        const float rk = fmaf(wu[0], rp[0], fmaf(wu[1], rp[1], wu[2] * rp[2]));
-        csum = fmaf(rk, W, csum);
+    if (nr & 1) {
+        csum = fmaf(a0, rp[0], fmaf(a1, rp[1], fmaf(a2, rp[2], csum)));
         sp[256] = csum;
     }
     scan[(nr + 2) * 256] = csum;
@@ -1202,6 +1203,9 @@ __device__ inline bool bp_column_view_fast(const BpView& v, const float* __restr
     const bool newton_ok = bp_newton_ok(col);
     const float dd = v.R - col.yr0;
     const float W = scale * (v.R * v.R) * __builtin_amdgcn_rcpf(dd * dd);
+    float wa[kMaxUTaps];
+#pragma unroll
+    for (int j = 0; j < kMaxUTaps; ++j) wa[j] = wu[j] * W;
     const float* wp = win + (kus - col0);
     bool left = false;
     bp_for_subs<ZC>([&](auto hc) {
@@ -1222,7 +1226,7 @@ __device__ inline bool bp_column_view_fast(const BpView& v, const float* __restr
             return;
         }
         // window offsets fit 24 bits: a full-rate multiply (the 32-bit one is quarter rate)
-        bp_row_integral3(wp + __mul24(ks - row0, stride), stride, nr, wu, W, scan);
+        bp_row_integral3(wp + __mul24(ks - row0, stride), stride, nr, wa, scan);
         bp_voxels_affine<kBpSub * h, false, false>(ch, ks, nr, scan, acc);
     });
     return left;
@@ -1252,6 +1256,9 @@ __device__ inline void bp_column_view(const BpView& v, const float* __restrict__
     }
     const float dd = v.R - col.yr0;
     const float W = scale * (v.R * v.R) * __builtin_amdgcn_rcpf(dd * dd);
+    float wa[kMaxUTaps];
+#pragma unroll
+    for (int j = 0; j < kMaxUTaps; ++j) wa[j] = wu[j] * W;
     const float rmin = col.rmin, rmax = col.rmax;
     const int cbase = kus - col0;
     const int ntaps = kue - kus + 1;
@@ -1301,18 +1308,18 @@ __device__ inline void bp_column_view(const BpView& v, const float* __restrict__
         if (zero_tol <= 0.0f && nr <= kMaxRows) {
             const float* rp = src + (FROM_LDS ? __mul24(ks - row0, stride) : (size_t)(ks - row0) * stride) + cbase;
             if (FROM_LDS && !__any(ntaps > kMaxUTaps)) {
-                bp_row_integral3(rp, stride, nr, wu, W, scan);
+                bp_row_integral3(rp, stride, nr, wa, scan);
             } else {
+                // taps from the last to the first, each weight times W, as bp_row_integral3 does
+                // (the same sums for at most three taps)
                 float csum = 0.0f;
                 scan[0] = 0.0f;
                 scan[256] = 0.0f;
                 for (int j = 0; j < nr; ++j) {
-                    float rk = 0.0f;
+                    for (int t = ntaps - 1; t >= kMaxUTaps; --t)
+                        csum = fmaf(dd_weight(kus + t, ku0, ku1, ulo, uhi, inv_u, zero_tol) * W, rp[t], csum);
 #pragma unroll
-                    for (int t = 0; t < kMaxUTaps; ++t) rk = fmaf(wu[t], rp[min(t, ntaps - 1)], rk);
-                    for (int t = kMaxUTaps; t < ntaps; ++t)
-                        rk = fmaf(dd_weight(kus + t, ku0, ku1, ulo, uhi, inv_u, zero_tol), rp[t], rk);
-                    csum = fmaf(rk, W, csum);
+                    for (int t = kMaxUTaps - 1; t >= 0; --t) csum = fmaf(wa[t], rp[min(t, ntaps - 1)], csum);
                     scan[(j + 2) * 256] = csum;
                     rp += stride;
                 }
