@@ -1107,19 +1107,21 @@ __device__ inline void bp_row_integral3(const float* __restrict__ rp, int stride
 // window contribute nothing).  Without it the footprints of all kBpSub voxels must lie inside
 // the table's rows (bp_sub_rows' `inside`: the sub-chunk is whole and its rows were not cut by
 // the detector or the window); the clamp then never binds, so both ways give the same sums.
-// Without EXACT, 1/(hi - lo) of voxel i comes by one Newton step from voxel i-1's: the
-// footprint height is affine in i and changes by a fraction d per voxel (the magnification
-// spread over one voxel), so the step leaves a relative error of about d^2 and costs two FMAs
-// instead of a reciprocal, which issues at half the FMA rate.  It is used only where
-// bp_newton_ok holds for the voxel column (d <= kNewtonRel, error below 4e-6; d is ~3.5e-4 at
-// 1000 mm from the source); with EXACT every voxel takes its own reciprocal.
-constexpr float kNewtonRel = 2e-3f;
+// Without EXACT, 1/(hi - lo) of voxel i comes from a quadratic in t = i - 7.5 (the sub-chunk's
+// middle): the footprint height is affine in i, h_i = h_m (1 + t e) with e the fraction it
+// changes by per voxel (the magnification spread over one voxel), and
+// 1/h_i = r_m (1 - t e + (t e)^2) up to a relative error below |t e|^3 <= (7.5 e)^3.  With t a
+// constant of the unrolled loop that is two FMAs per voxel and no reciprocal (which issues at
+// half the FMA rate).  It is used only where bp_series_ok holds for the voxel column
+// (e <= kSeriesRel, error below 3.4e-6; e is ~3.5e-4 at 1000 mm from the source for 0.25 mm
+// voxels); with EXACT every voxel takes its own reciprocal.
+constexpr float kSeriesRel = 2e-3f;
 
 // The per-voxel change d of the footprint height of any affine sub-chunk of a column: with
 // h_0 = |Ahi - Alo| >= |Bv| hz (rmax + rmin) and |Dhi - Dlo| = |Bv| 2 hz (rmax - rmin) on either
 // side of the source height, d <= 2 (rmax - rmin) / (rmax + rmin), a bound of the column alone.
-__device__ inline bool bp_newton_ok(const BpColumn& col) {
-    return 2.0f * (col.rmax - col.rmin) <= kNewtonRel * (col.rmax + col.rmin);
+__device__ inline bool bp_series_ok(const BpColumn& col) {
+    return 2.0f * (col.rmax - col.rmin) <= kSeriesRel * (col.rmax + col.rmin);
 }
 
 template <int OFF, bool CLAMP, bool EXACT, int ZC>
@@ -1128,14 +1130,22 @@ __device__ inline void bp_voxels_affine(const BpChunk& ch, int ks, int nr, const
     static_assert(OFF + kBpSub <= ZC, "sub-chunk inside the work-item");
     const float base = (float)(ks - 1), top = (float)(nr + 1);
     const float alo = ch.Alo - base, ahi = ch.Ahi - base;
-    float rinv = __builtin_amdgcn_rcpf(ahi - alo);
+    // 1/h_i = rm + t c1 + t^2 c2 with t = i - 7.5, rm = 1/h(7.5), e = (Dhi - Dlo) rm,
+    // c1 = -rm e, c2 = rm e^2
+    constexpr float kMid = 0.5f * (float)(kBpSub - 1);
+    const float dD = ch.Dhi - ch.Dlo;
+    const float rm = EXACT ? 0.0f : __builtin_amdgcn_rcpf(fmaf(kMid, dD, ahi - alo));
+    const float e = dD * rm;
+    const float c1 = -(rm * e), c2 = -(c1 * e);
 #pragma unroll
     for (int i = 0; i < kBpSub; ++i) {
         const float lo = fmaf((float)i, ch.Dlo, alo), hi = fmaf((float)i, ch.Dhi, ahi);
+        const float t = (float)i - kMid;
+        float rinv;
         if (EXACT)
             rinv = __builtin_amdgcn_rcpf(hi - lo);
-        else if (i > 0)
-            rinv *= fmaf(lo - hi, rinv, 2.0f);
+        else
+            rinv = fmaf(t * t, c2, fmaf(t, c1, rm));
         const float xl = CLAMP ? __builtin_amdgcn_fmed3f(lo, 1.0f, top) : lo;
         const float xh = CLAMP ? __builtin_amdgcn_fmed3f(hi, 1.0f, top) : hi;
         acc[OFF + i] = fmaf(bp_lookup(scan, xh) - bp_lookup(scan, xl), rinv, acc[OFF + i]);
@@ -1172,7 +1182,7 @@ __device__ inline void bp_for_subs(F&& f) {
 
 // The fast pass (kBpFast): the common case only -- window in LDS, sub-chunk on one side of the
 // source height, at most kMaxRows rows and kMaxUTaps columns, a footprint height that changes
-// slowly enough along z for the Newton-step normaliser (bp_newton_ok), no weight zeroing (the
+// slowly enough along z for the series normaliser (bp_series_ok), no weight zeroing (the
 // host runs this pass only with zero_tol = 0).  The column's u-footprint, u-weights and FDK weight are
 // worked out once per view for all ZC voxels; every sub-chunk then builds its own row integral.
 // Returns true when it leaves a sub-chunk of this (work-item, view) to the completing pass,
@@ -1200,7 +1210,7 @@ __device__ inline bool bp_column_view_fast(const BpView& v, const float* __restr
         wu[j] = (kus + j <= kue) ? fmaxf(w, 0.0f) : 0.0f;
     }
     if (col.ku0 == col.ku1) wu[0] = 1.0f;
-    const bool newton_ok = bp_newton_ok(col);
+    const bool series_ok = bp_series_ok(col);
     const float dd = v.R - col.yr0;
     const float W = scale * (v.R * v.R) * __builtin_amdgcn_rcpf(dd * dd);
     float wa[kMaxUTaps];
@@ -1221,7 +1231,7 @@ __device__ inline bool bp_column_view_fast(const BpView& v, const float* __restr
         bool inside;
         bp_sub_rows(ch, n, row0, nrows, nv, ks, nr, inside);
         if (nr <= 0) return;
-        if (nr > kMaxRows || !taps_ok || !inside || !newton_ok) {
+        if (nr > kMaxRows || !taps_ok || !inside || !series_ok) {
             left = true;
             return;
         }
@@ -1302,7 +1312,7 @@ __device__ inline void bp_column_view(const BpView& v, const float* __restrict__
         if constexpr (KIND == kBpRest) {
             // the sub-chunks the fast pass did
             const bool common = window_in_lds && lin && inside && nr <= kMaxRows && ntaps <= kMaxUTaps &&
-                                zero_tol <= 0.0f && bp_newton_ok(col);
+                                zero_tol <= 0.0f && bp_series_ok(col);
             if (common) return;
         }
         if (zero_tol <= 0.0f && nr <= kMaxRows) {

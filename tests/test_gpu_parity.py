@@ -53,7 +53,7 @@ CASES = {
     "off_center": dict(NumberOfImage=[24, 20, 18], ReconCenter=[6.0, -5.0, 4.0], NumberOfDetectorPixels=[48, 40],
                        NumberOfViews=12, StartAngle=0.2, EndAngle=0.2 + 2 * np.pi),
     # the source 60 mm from the isocentre: a voxel column's footprint height changes by ~2.4 %
-    # per voxel along z, too fast for the backprojector's Newton-step normaliser, so its
+    # per voxel along z, too fast for the backprojector's series normaliser, so its
     # sub-chunks take exact reciprocals in the completing pass
     "near_source": dict(SourceInit=[0, 60.0, 0], DetectorInit=[0, -40.0, 0], NumberOfImage=[16, 16, 32],
                         ImagePixelSpacing=[1.0, 1.0, 1.0], NumberOfDetectorPixels=[20, 40],
