@@ -41,7 +41,8 @@ def _rel(a, b):
 @pytest.mark.parametrize("name", ["flat", "flat_45deg_ties", "flat_offsets", "anisotropic", "helical",
                                   "curved", "wide_footprint", "curved_fan_2d", "near_source", "off_center",
                                   "reverse", "helical_reverse", "short_scan", "source_z", "origin",
-                                  "single_view", "thin_z"])
+                                  "single_view", "thin_z", "curved_helical", "curved_offsets", "curved_reverse",
+                                  "curved_near_source", "aniso_helical", "flat_fan_2d"])
 def test_adjoint_matches_oracle_transpose(dev, name):
     p = make_params(**CASES[name])
     g = O.Geometry(p)

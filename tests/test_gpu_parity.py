@@ -78,6 +78,23 @@ CASES = {
                         StartAngle=0.7, EndAngle=0.7 + 2 * np.pi),
     "thin_z": dict(NumberOfImage=[24, 24, 2], NumberOfDetectorPixels=[40, 32], NumberOfViews=12,
                    StartAngle=0.1, EndAngle=0.1 + 2 * np.pi),
+    # the curved detector with the other features (its u offset is an angle, in radians)
+    "curved_helical": dict(NumberOfImage=[20, 20, 24], NumberOfDetectorPixels=[48, 24], NumberOfViews=16,
+                           Pitch=0.5, EndAngle=4 * np.pi, DetectorShape="Curved"),
+    "curved_offsets": dict(NumberOfImage=[22, 18, 14], NumberOfDetectorPixels=[47, 29], NumberOfViews=10,
+                           StartAngle=0.3, EndAngle=0.3 + 2 * np.pi, ReconCenter=[0.7, -0.4, 0.3],
+                           DetectorOffset=[0.02, -0.6], DetectorShape="Curved"),
+    "curved_reverse": dict(NumberOfImage=[24, 24, 16], NumberOfDetectorPixels=[48, 30], NumberOfViews=12,
+                           StartAngle=0.1 + 2 * np.pi, EndAngle=0.1, DetectorShape="Curved"),
+    "curved_near_source": dict(SourceInit=[0, 60.0, 0], DetectorInit=[0, -40.0, 0], NumberOfImage=[16, 16, 32],
+                               ImagePixelSpacing=[1.0, 1.0, 1.0], NumberOfDetectorPixels=[40, 40],
+                               DetectorPixelSize=[2.0, 2.0], NumberOfViews=12, StartAngle=0.1,
+                               EndAngle=0.1 + 2 * np.pi, DetectorShape="Curved"),
+    "aniso_helical": dict(NumberOfImage=[26, 18, 20], ImagePixelSpacing=[0.4, 0.6, 0.5],
+                          NumberOfDetectorPixels=[45, 23], DetectorPixelSize=[0.9, 1.3], NumberOfViews=18,
+                          Pitch=0.7, StartAngle=-0.2, EndAngle=4 * np.pi - 0.2),
+    "flat_fan_2d": dict(NumberOfImage=[48, 48, 1], ImagePixelSpacing=[0.5, 0.5, 0.5],
+                        NumberOfDetectorPixels=[96, 1], DetectorPixelSize=[0.5, 0.75], NumberOfViews=24),
 }
 
 
