@@ -1918,16 +1918,26 @@ __device__ inline void dft_reg(float2 (&v)[R], bool inv) {
     }
 }
 
-// One radix-R Stockham stage over sub-transforms of length ns: butterfly j reads
-// buf[j + r N/R], applies the twiddles exp(-+2 pi i r (j mod ns) / (ns R)) (stage-major table of
-// twiddle_kernel), takes an R-point DFT and writes buf[(j / ns) ns R + j mod ns + r ns], with T
-// threads per workgroup.  All inputs are read into registers before a barrier, then written, so
-// one LDS buffer suffices.
 // largest power of two not above r (r >= 1)
 constexpr int top_bit(int r) { return (r & (r - 1)) == 0 ? r : top_bit(r & (r - 1)); }
 
-template <int R, int N, int T>
-__device__ void stockham_stage(float2* buf, const float2* __restrict__ tw, int ns, bool inv) {
+// One radix-R Stockham stage over sub-transforms of length ns, with T threads per workgroup:
+// butterfly j reads element j + r N/R, applies the twiddles exp(-+2 pi i r (j mod ns) / (ns R))
+// (stage-major table of twiddle_kernel; the power-of-two ones loaded, the others formed as
+// products of two earlier ones, at most three roundings deep), takes an R-point DFT and writes
+// element (j / ns) ns R + j mod ns + r ns.  Elements come from the LDS buffer or, for the
+// filter's first forward stage, from a global-memory loader; they go to the LDS buffer, to it
+// times the spectrum (the filter's last forward stage) or, for its last inverse stage, to a
+// global-memory writer.  All inputs are read into registers before a barrier, then written, so
+// one LDS buffer suffices:
+//   SRC 0: buf, SRC 1: load(n) (no barrier before the writes: nothing in buf is read);
+//   DST 0: buf, DST 1: buf times spec[n], DST 2: store(n, value) (no barrier at all).
+constexpr int kFromLds = 0, kFromGlobal = 1;
+constexpr int kToLds = 0, kToLdsTimesSpectrum = 1, kToGlobal = 2;
+
+template <int R, int N, int T, int SRC, int DST, typename Load, typename Store>
+__device__ void stockham_stage_io(float2* buf, const float2* __restrict__ tw, int ns, bool inv,
+                                  const double* __restrict__ spec, Load&& load, Store&& store) {
     constexpr int NBF = N / R;  // butterflies
     constexpr int NB = (NBF + T - 1) / T;
     const int tid = threadIdx.x;
@@ -1937,12 +1947,9 @@ __device__ void stockham_stage(float2* buf, const float2* __restrict__ tw, int n
         const int j = tid + b * T;
         if (j < NBF) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[b][r] = buf[j + r * NBF];
+            for (int r = 0; r < R; ++r) v[b][r] = SRC == kFromGlobal ? load(j + r * NBF) : buf[j + r * NBF];
             const int m = j % ns;
             if (m != 0) {
-                // twiddles W^r for r a power of two from the table, the others as products of
-                // two earlier ones (at most three roundings deep): fewer bytes through the
-                // texture path, which is busy nearly all the time in this kernel
                 float2 w[R];
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
@@ -1960,38 +1967,57 @@ __device__ void stockham_stage(float2* buf, const float2* __restrict__ tw, int n
             dft_reg<R>(v[b], inv);
         }
     }
-    __syncthreads();
+    if constexpr (SRC == kFromLds && DST != kToGlobal) __syncthreads();
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
         const int j = tid + b * T;
         if (j < NBF) {
             const int base = (j / ns) * ns * R + (j % ns);
 #pragma unroll
-            for (int r = 0; r < R; ++r) buf[base + r * ns] = v[b][r];
+            for (int r = 0; r < R; ++r) {
+                const int n = base + r * ns;
+                if constexpr (DST == kToGlobal) {
+                    store(n, v[b][r]);
+                } else if constexpr (DST == kToLdsTimesSpectrum) {
+                    const float h = (float)spec[n];
+                    buf[n] = make_float2(v[b][r].x * h, v[b][r].y * h);
+                } else {
+                    buf[n] = v[b][r];
+                }
+            }
         }
     }
-    __syncthreads();
+    if constexpr (DST != kToGlobal) __syncthreads();
 }
 
-// In-LDS Stockham FFT of length 2^LOG2N: radix-16 stages (two radix-4 levels in registers per
-// LDS round trip), then one radix-2, 4 or 8 stage for the remaining factor.  `inv` selects the
-// conjugate twiddles (unnormalised inverse).
-template <int LOG2N>
-__device__ void fft_lds(float2* buf, const float2* __restrict__ tw, bool inv) {
+// Stage st of the length-2^LOG2N transform (radix 16 for st < LOG2N / 4, then the radix of the
+// remaining factor), with its source and sink.
+template <int LOG2N, int ST, int SRC, int DST, typename Load, typename Store>
+__device__ void fft_stage_io(float2* buf, const float2* __restrict__ tw, bool inv, const double* __restrict__ spec,
+                             Load&& load, Store&& store) {
     constexpr int N = 1 << LOG2N;
     constexpr int T = fft_threads(LOG2N);
-    constexpr int kRest = LOG2N % 4;
-    int ns = 1;
-    if constexpr (LOG2N >= 4) {
-#pragma unroll
-        for (int st = 0; st < LOG2N / 4; ++st) {
-            stockham_stage<16, N, T>(buf, tw, ns, inv);
-            ns *= 16;
-        }
+    constexpr int n16 = LOG2N / 4;
+    constexpr int R = ST < n16 ? 16 : 1 << (LOG2N % 4);
+    const int ns = 1 << (4 * ST);
+    stockham_stage_io<R, N, T, SRC, DST>(buf, tw, ns, inv, spec, load, store);
+}
+
+// The filter's convolution: forward FFT from `load`, times the spectrum, inverse FFT to `store`,
+// with the first forward stage reading global memory, the spectrum applied as the last forward
+// stage writes LDS and the last inverse stage writing global memory.
+template <int LOG2N, int ST = 0, typename Load, typename Store>
+__device__ void fft_filter_stages(float2* buf, const float2* __restrict__ tw, const double* __restrict__ spec,
+                                  Load&& load, Store&& store) {
+    constexpr int NST = LOG2N / 4 + (LOG2N % 4 ? 1 : 0);  // stages per transform
+    if constexpr (ST < 2 * NST) {
+        constexpr bool inv = ST >= NST;
+        constexpr int k = inv ? ST - NST : ST;
+        constexpr int SRC = ST == 0 ? kFromGlobal : kFromLds;
+        constexpr int DST = ST == NST - 1 ? kToLdsTimesSpectrum : (ST == 2 * NST - 1 ? kToGlobal : kToLds);
+        fft_stage_io<LOG2N, k, SRC, DST>(buf, tw, inv, spec, load, store);
+        fft_filter_stages<LOG2N, ST + 1>(buf, tw, spec, load, store);
     }
-    if constexpr (kRest == 1) stockham_stage<2, N, T>(buf, tw, ns, inv);
-    if constexpr (kRest == 2) stockham_stage<4, N, T>(buf, tw, ns, inv);
-    if constexpr (kRest == 3) stockham_stage<8, N, T>(buf, tw, ns, inv);
 }
 
 // Filtering() / filter_proj (python/Reconstruction_pycuda.py:223-251): two detector rows per
@@ -2002,6 +2028,11 @@ __device__ void fft_lds(float2* buf, const float2* __restrict__ tw, bool inv) {
 // ConeBeamGPU's filtering (python/ConeBeamGPU.py:296-302) weighs at cell centres (off_u =
 // du R/(2 SDD), off_v = -dv R/(2 SDD)) and filters fliplr(row): the spectrum is real and
 // even, so that equals the mirror image of the filtered row, written with mirror = 1.
+// The FFTs are Stockham transforms in LDS, radix-16 stages (two radix-4 levels in registers per
+// LDS round trip) and one radix-2, 4 or 8 stage for the remaining factor.  The first forward
+// stage reads the two rows from global memory and the last inverse stage writes them back, so the
+// LDS sees a write from every stage but the last and a read from every stage but the first, and
+// no separate load, spectrum or store pass.
 // `in` and `out` may be the same buffer (in-place filtering): a block reads its two rows before
 // its first barrier and writes them after its last one, and no other block touches them.
 template <int LOG2N>
@@ -2017,7 +2048,8 @@ __global__ void __launch_bounds__(fft_threads(LOG2N)) fdk_filter_kernel(
     const bool hasB = rowB < n_rows;
     const float pa = ((float)(nv / 2.0f) - (float)(rowA % nv)) * sv + ov;
     const float pb = ((float)(nv / 2.0f) - (float)(rowB % nv)) * sv + ov;
-    for (int n = threadIdx.x; n < N; n += T) {
+    // sample n of the weighted, zero-padded row pair (read by the first forward stage)
+    auto load = [&](int n) {
         float2 val = make_float2(0.0f, 0.0f);
         if (n < nu) {
             const float kk = ((float)n - (float)nu / 2.0f) * su + ou;
@@ -2028,22 +2060,17 @@ __global__ void __launch_bounds__(fft_threads(LOG2N)) fdk_filter_kernel(
                 val.y = in[(size_t)rowB * nu + n] * wb;
             }
         }
-        buf[n] = val;
-    }
-    __syncthreads();
-    fft_lds<LOG2N>(buf, tw, false);
-    for (int n = threadIdx.x; n < N; n += T) {
-        const float h = (float)spec[n];
-        buf[n] = make_float2(buf[n].x * h, buf[n].y * h);
-    }
-    __syncthreads();
-    fft_lds<LOG2N>(buf, tw, true);
-    for (int n = threadIdx.x; n < nu; n += T) {
-        const float2 val = buf[n];
-        const int j = mirror ? nu - 1 - n : n;
-        out[(size_t)rowA * nu + j] = val.x * out_scale;
-        if (hasB) out[(size_t)rowB * nu + j] = val.y * out_scale;
-    }
+        return val;
+    };
+    // sample n of the filtered pair (written by the last inverse stage; the padding is dropped)
+    auto store = [&](int n, float2 val) {
+        if (n < nu) {
+            const int j = mirror ? nu - 1 - n : n;
+            out[(size_t)rowA * nu + j] = val.x * out_scale;
+            if (hasB) out[(size_t)rowB * nu + j] = val.y * out_scale;
+        }
+    };
+    fft_filter_stages<LOG2N>(buf, tw, spec, load, store);
 }
 
 template <int L>
