@@ -131,6 +131,34 @@ def test_full_size_dd_fdk_matches_c_oracle_on_whole_volume(dev):
     assert err_f <= TOL and max(errs.values()) <= TOL
 
 
+def test_full_size_complete_fdk_matches_c_oracle(dev):
+    """The benchmark's whole reconstruction against the oracle: the 40-ellipsoid phantom's 720
+    projections filtered and backprojected into every voxel of the 512^3 volume by the GPU
+    (ctr.fdk) and by the oracle's batched filter and C restatement of its backprojection (FP64,
+    every host core; about a minute on 16 cores), from the same raw projections."""
+    import ctreconstruction_amd as ctr
+    from ctreconstruction_amd import phantom
+    from oracle import dd_c as C
+
+    if not os.path.exists(C.LIB_PATH):
+        pytest.fail("oracle/lib/liboracle.so is not built (oracle.dd_c.build())")
+    geom = ctr.ScanGeometry(DD_FULL)
+    g = O.Geometry(DD_FULL)
+    ell = phantom.random_ellipsoids(40, phantom.half_extent(geom), 3)
+    vol = phantom.ellipsoid_volume(geom, ell, device=dev) * phantom.MU_WATER
+    proj = ctr.dd_forward(vol, geom)
+    del vol
+    rec = ctr.fdk(proj, geom).cpu().numpy()
+    raw = proj.cpu().numpy()
+    del proj
+    q_ref = C.filter_projections(g, raw)
+    del raw
+    ref = C.backproject(g, q_ref, "ref")
+    err = float(np.abs(rec - ref).max() / np.abs(ref).max())
+    print(f"DD FDK 512^3, all 720 views into the whole volume: max rel err {err:.2e}")
+    assert err <= TOL
+
+
 def test_full_size_pixel_fdk_matches_oracle_on_two_views(dev):
     from ctreconstruction_amd.pixel_fdk import PixelScan, pixel_backproject, pixel_filter
     scan = PixelScan(PIXEL_FULL)
@@ -217,6 +245,30 @@ def test_c2_workload_matches_oracle(dev):
           f"FDK rel. RMSE vs phantom (central 16 slices) {rel_rmse:.3f}")
     assert e_fp <= TOL and e_f <= TOL and e_bp <= TOL
     assert rel_rmse < 0.5
+
+
+def test_c2_complete_fdk_matches_c_oracle(dev):
+    """The whole C2 reconstruction against the oracle: the phantom's 360 projections filtered and
+    backprojected into every voxel of the 256^3 volume by the GPU (ctr.fdk) and by the oracle's
+    batched filter and C restatement of its backprojection (FP64, every host core), from the same
+    raw projections."""
+    import ctreconstruction_amd as ctr
+    from ctreconstruction_amd import phantom
+    from oracle import dd_c as C
+
+    if not os.path.exists(C.LIB_PATH):
+        pytest.fail("oracle/lib/liboracle.so is not built (oracle.dd_c.build())")
+    geom = ctr.ScanGeometry(DD_C2)
+    g = O.Geometry(DD_C2)
+    ell = phantom.random_ellipsoids(20, phantom.half_extent(geom), 2)
+    vol_t = phantom.ellipsoid_volume(geom, ell, device=dev) * phantom.MU_WATER
+    proj = ctr.dd_forward(vol_t, geom)
+    rec = ctr.fdk(proj, geom).cpu().numpy()
+    q_ref = C.filter_projections(g, proj.cpu().numpy())
+    ref = C.backproject(g, q_ref, "ref")
+    err = float(np.abs(rec - ref).max() / np.abs(ref).max())
+    print(f"C2 FDK, all 360 views into the whole 256^3 volume: max rel err {err:.2e}")
+    assert err <= TOL
 
 
 def test_c2_central_rays_and_linearity(dev):
