@@ -1,6 +1,7 @@
-"""ctypes front end of the C restatement of the oracle's FDK backprojection (``c/dd_backproject.c``)
-and a multi-threaded restatement of its filter; the library also holds the pixel oracle's
-backprojection (``c/pixel_backproject.c``, front end in :mod:`oracle.pixel_c`).
+"""ctypes front end of the C restatements of the oracle's FDK backprojection (``c/dd_backproject.c``)
+and DD forward projection (``c/dd_forward.c``), and a multi-threaded restatement of its filter;
+the library also holds the pixel oracle's backprojection (``c/pixel_backproject.c``, front end in
+:mod:`oracle.pixel_c`).
 
 TEST INFRASTRUCTURE: used by the tests, the smoke check and the benchmark's CPU-baseline leg as
 the checker and as the CPU time of a compiled port; the product path never imports it.
@@ -9,7 +10,9 @@ the checker and as the CPU time of a compiled port; the product path never impor
 same FP64 operations in the same order; it takes the detector grid, voxel centres and per-view
 source heights from :mod:`oracle.dd_oracle`), split over OpenMP threads.  ``filter_projections``
 is :func:`oracle.dd_oracle.filter_projections` with its FFTs batched over views and run by
-``scipy.fft`` on several workers.
+``scipy.fft`` on several workers.  ``forward_project_view`` returns what
+:func:`oracle.dd_oracle.forward_project_view` returns (the slab loop in C, the view's line tables
+from :func:`oracle.dd_oracle.fp_view_tables`), its cells split over OpenMP threads.
 """
 from __future__ import annotations
 
@@ -22,7 +25,7 @@ import numpy as np
 from . import dd_oracle as O
 
 ORACLE_DIR = os.path.dirname(os.path.abspath(__file__))
-SOURCES = [os.path.join(ORACLE_DIR, "c", "dd_backproject.c"), os.path.join(ORACLE_DIR, "c", "pixel_backproject.c")]
+SOURCES = [os.path.join(ORACLE_DIR, "c", n) for n in ("dd_backproject.c", "dd_forward.c", "pixel_backproject.c")]
 LIB_PATH = os.path.join(ORACLE_DIR, "lib", "liboracle.so")
 CFLAGS = ["-O3", "-fopenmp", "-ffp-contract=off", "-fPIC", "-shared", "-std=c11", "-Wall", "-Wextra"]
 
@@ -55,6 +58,12 @@ def lib():
             ctypes.c_double, dp, dp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
             dp, dp, dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
             ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, dp]
+        i64 = ctypes.c_int64
+        L.ddo_forward_view.restype = ctypes.c_int
+        L.ddo_forward_view.argtypes = [
+            ctypes.POINTER(ctypes.c_float), i64, i64, i64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+            dp, dp, dp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+            dp, dp, dp, dp, dp, dp, dp, dp, dp, i64, ctypes.c_double, ctypes.c_int, dp]
         L.pxo_backproject.restype = ctypes.c_int
         L.pxo_backproject.argtypes = [
             dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp, ctypes.c_double, ctypes.c_double,
@@ -103,6 +112,39 @@ def backproject(g: O.Geometry, q: np.ndarray, mode: str = "ref", zero_tol: float
     if rc != 0:
         raise RuntimeError(f"ddo_backproject failed with code {rc}")
     return out
+
+
+def forward_project_view(g: O.Geometry, vol: np.ndarray, i: int, zero_tol: float | None = 1e-6,
+                         threads: int | None = None) -> np.ndarray:
+    """View ``i`` of the DD forward projection of ``vol`` [nz, ny, nx] -> [nv, nu] float64, as
+    :func:`oracle.dd_oracle.forward_project_view` (FP64; the volume's float32 values)."""
+    threads = host_threads() if threads is None else int(threads)
+    v = np.ascontiguousarray(vol, dtype=np.float32)
+    if v.shape != (g.nz, g.ny, g.nx):
+        raise ValueError(f"volume {v.shape} != ({g.nz}, {g.ny}, {g.nx})")
+    br, *tabs = O.fp_view_tables(g, i)
+    tabs = [np.ascontiguousarray(np.broadcast_to(t, (g.nv, g.nu)), dtype=np.float64).ravel() for t in tabs]
+    sa, ta, va = O.AXES[br]
+    strides = (1, g.nx, g.nx * g.ny)
+    planes = [np.ascontiguousarray(p, dtype=np.float64) for p in g.planes()]
+    dvec = (g.dx, -g.dy, -g.dz)
+    ps, pt, pv = planes[sa], planes[ta], planes[va]
+    out = np.zeros(g.nv * g.nu, dtype=np.float64)
+    rc = lib().ddo_forward_view(
+        v.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), strides[sa], strides[ta], strides[va],
+        len(ps) - 1, len(pt) - 1, len(pv) - 1, _dp(ps), _dp(pt), _dp(pv),
+        float(dvec[sa]), float(dvec[ta]), float(dvec[va]), *[_dp(t) for t in tabs], g.nv * g.nu,
+        0.0 if zero_tol is None else float(zero_tol), threads, _dp(out))
+    if rc != 0:
+        raise RuntimeError(f"ddo_forward_view failed with code {rc}")
+    return out.reshape(g.nv, g.nu)
+
+
+def forward_project(g: O.Geometry, vol: np.ndarray, views=None, zero_tol: float | None = 1e-6,
+                    threads: int | None = None) -> np.ndarray:
+    """``views`` (default all) of the DD forward projection -> [len(views), nv, nu] float64."""
+    views = list(range(g.nview)) if views is None else [int(x) for x in views]
+    return np.stack([forward_project_view(g, vol, i, zero_tol, threads) for i in views])
 
 
 def filter_projections(g: O.Geometry, proj: np.ndarray, threads: int | None = None,

@@ -75,6 +75,29 @@ def test_full_size_forward_central_rays_and_linearity(dev):
                                atol=2e-5 * float(proj.max()))
 
 
+def test_full_size_forward_matches_c_oracle_on_eight_views(dev):
+    """The benchmark phantom's forward projection at 512^3 / 720 views / 1024 x 768, every cell
+    of eight views spread over the scan (view 90 at exactly 45 degrees, where the slab branch
+    ties), against the oracle's C restatement of its slab loop (FP64, every host core; the
+    oracle's 1e-6 weight zeroing)."""
+    import ctreconstruction_amd as ctr
+    from ctreconstruction_amd import phantom
+    from oracle import dd_c as C
+
+    if not os.path.exists(C.LIB_PATH):
+        pytest.fail("oracle/lib/liboracle.so is not built (oracle.dd_c.build())")
+    geom = ctr.ScanGeometry(DD_FULL)
+    g = O.Geometry(DD_FULL)
+    ell = phantom.random_ellipsoids(40, phantom.half_extent(geom), 3)
+    vol = phantom.ellipsoid_volume(geom, ell, device=dev) * phantom.MU_WATER
+    views = [0, 90, 181, 270, 359, 450, 541, 631]
+    got = ctr.dd_forward(vol, geom)[views].cpu().numpy()
+    ref = C.forward_project(g, vol.cpu().numpy(), views=views)
+    err = float(np.abs(got - ref).max() / np.abs(ref).max())
+    print(f"DD forward 512^3, eight whole views: max rel err {err:.2e}")
+    assert err <= TOL
+
+
 def test_full_size_dd_fdk_matches_oracle_on_two_views(dev):
     import ctreconstruction_amd as ctr
     geom = ctr.ScanGeometry(DD_FULL)
