@@ -271,7 +271,8 @@ def test_c2_workload_matches_oracle(dev):
 
 
 def test_c2_complete_fdk_matches_c_oracle(dev):
-    """The whole C2 reconstruction against the oracle: the phantom's 360 projections filtered and
+    """The forward projection over every cell of every 30th view, and the whole C2
+    reconstruction against the oracle: the phantom's 360 projections filtered and
     backprojected into every voxel of the 256^3 volume by the GPU (ctr.fdk) and by the oracle's
     batched filter and C restatement of its backprojection (FP64, every host core), from the same
     raw projections."""
@@ -286,12 +287,17 @@ def test_c2_complete_fdk_matches_c_oracle(dev):
     ell = phantom.random_ellipsoids(20, phantom.half_extent(geom), 2)
     vol_t = phantom.ellipsoid_volume(geom, ell, device=dev) * phantom.MU_WATER
     proj = ctr.dd_forward(vol_t, geom)
+    # the forward projection over every cell of every 30th view
+    views = list(range(0, g.nview, 30))
+    fp_ref = C.forward_project(g, vol_t.cpu().numpy(), views=views)
+    e_fp = float(np.abs(proj[views].cpu().numpy() - fp_ref).max() / np.abs(fp_ref).max())
     rec = ctr.fdk(proj, geom).cpu().numpy()
     q_ref = C.filter_projections(g, proj.cpu().numpy())
     ref = C.backproject(g, q_ref, "ref")
     err = float(np.abs(rec - ref).max() / np.abs(ref).max())
-    print(f"C2 FDK, all 360 views into the whole 256^3 volume: max rel err {err:.2e}")
-    assert err <= TOL
+    print(f"C2 forward, every cell of 12 views: max rel err {e_fp:.2e}; "
+          f"FDK, all 360 views into the whole 256^3 volume: max rel err {err:.2e}")
+    assert e_fp <= TOL and err <= TOL
 
 
 def test_c2_central_rays_and_linearity(dev):
