@@ -13,6 +13,8 @@ Prints one JSON line.  ``value`` is seconds per FDK reconstruction (lower is bet
 with algorithmic bytes 4*(views*nv*nu + nx*ny*nz) per launch; ``cpu_baseline`` times the oracle's C
 restatement (FP64, OpenMP on every host core) on 8 views spread over the scan into the whole
 volume and scales the wall time to 720 views; a one-thread run of one view is reported beside it.
+The forward projection's own CPU baseline (``dd_forward.cpu_baseline``) times the C restatement
+of the oracle's slab loop on every cell of 4 views and scales it the same way.
 
 ``pixel_fdk`` times the reference's other FDK, the pixel-driven one (voxel-driven bilinear
 backprojection, python/ConeBeamGPU.py:203-321), on the reference's geometry file
