@@ -120,6 +120,14 @@ __global__ void px_fill_kernel(int* __restrict__ p, int n) {
     if (i < n) p[i] = 0;
 }
 
+// The reference's inclusive row test 0 <= fv <= v_last (python/ConeBeamGPU.py:80-102) as one
+// unsigned compare of the bit patterns: non-negative floats order like their bits, and negative
+// values and NaN have patterns above any finite v_last >= 0.  The one value the two forms treat
+// differently, -0.0, never occurs: an FMA rounds a zero sum to +0 unless both of its terms are
+// -0.0, fv is an FMA onto fvb, and fvb is an FMA onto v_mid = (nv - 1) / 2 >= +0 (or an
+// integer-valued float), so neither is ever -0.0.
+__device__ inline bool px_row_ok(float fv, unsigned vlast_bits) { return __float_as_uint(fv) <= vlast_bits; }
+
 // Window of a (tile, view) pair: detector columns floor(fu_lo)-1 .. floor(fu_hi)+2 and rows
 // likewise from the extremes of V over the tile's z range (all values uniform in the group).
 struct PxWindow {
@@ -183,6 +191,7 @@ __global__ void __launch_bounds__(kPxThreads) pixel_backproject_kernel(
     const int view_bytes = (int)(view_cells * 4);
 
     float acc[TZ];
+    const unsigned vlast_bits = __float_as_uint(a.v_last);
 #pragma unroll
     for (int k = 0; k < TZ; ++k) acc[k] = 0.0f;
 
@@ -268,7 +277,7 @@ __global__ void __launch_bounds__(kPxThreads) pixel_backproject_kernel(
             const float m0 = fmaf(fx, q01 - q00, q00);
             const float m1 = fmaf(fx, q11 - q10, q10);
             const float val = fmaf(fy, m1 - m0, m0);
-            const float wk = (fv >= 0.0f && fv <= a.v_last) ? wc : 0.0f;
+            const float wk = px_row_ok(fv, vlast_bits) ? wc : 0.0f;
             acc[k] = fmaf(val, wk, acc[k]);
         }
     }
@@ -308,6 +317,7 @@ __global__ void __launch_bounds__(kPxThreads) pixel_backproject_rest_kernel(
     const size_t view_cells = (size_t)a.nu * a.nv;
 
     float acc[TZ];
+    const unsigned vlast_bits = __float_as_uint(a.v_last);
 #pragma unroll
     for (int k = 0; k < TZ; ++k) acc[k] = 0.0f;
     for (int v = 0; v < a.n_views; ++v) {
@@ -327,7 +337,7 @@ __global__ void __launch_bounds__(kPxThreads) pixel_backproject_rest_kernel(
 #pragma unroll
         for (int k = 0; k < TZ; ++k) {
             const float fv = fmaf((float)k, kz, fvb);
-            const bool ok = fv >= 0.0f && fv <= a.v_last;
+            const bool ok = px_row_ok(fv, vlast_bits);
             const int iv = min(max((int)floorf(fv), 0), a.nv - 2);
             const float fy = __saturatef(fv - (float)iv);
             const float* r0p = qv + (size_t)iv * a.nu + iu;
